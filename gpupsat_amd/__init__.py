@@ -1,0 +1,188 @@
+"""gpupsat_amd -- MI355X SAT engine: Python binding over the C ABI of libgpsat.so.
+
+The compute path is the HIP library built from gpupsat_amd/csrc for gfx950; this
+module only marshals numpy arrays through ctypes.  There is no CPU fallback: if the
+library is missing, or no GPU is visible, calls raise GpsatError.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from typing import Optional
+
+import numpy as np
+
+from .formula import Batch, Formula, pack_rows, unpack_rows, words_per_row  # noqa: F401
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libgpsat.so")
+CLI_PATH = os.path.join(_HERE, "bin", "gpupsat")
+HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "gpsat.h")
+
+SAT, UNSAT, UNDEF = 0, 1, 2
+
+
+class GpsatError(RuntimeError):
+    pass
+
+
+_lib = None
+
+
+def lib():
+    """Load libgpsat.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise GpsatError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+        _lib = ctypes.CDLL(LIB_PATH)
+        _lib.gpsat_last_error.restype = ctypes.c_char_p
+        _lib.gpsat_version.restype = ctypes.c_char_p
+    return _lib
+
+
+def _check(rc: int):
+    if rc != 0:
+        raise GpsatError(f"gpsat error {rc}: {lib().gpsat_last_error().decode()}")
+
+
+def _p(a: np.ndarray):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def device_count() -> int:
+    c = ctypes.c_int32(0)
+    _check(lib().gpsat_device_count(ctypes.byref(c)))
+    return c.value
+
+
+def synchronize():
+    _check(lib().gpsat_device_synchronize())
+
+
+class DevBuf:
+    """A device allocation owned by Python (freed on close / garbage collection)."""
+
+    def __init__(self, nbytes: int):
+        self.nbytes = int(nbytes)
+        p = ctypes.c_void_p()
+        _check(lib().gpsat_dev_alloc(ctypes.c_int64(self.nbytes), ctypes.byref(p)))
+        self.ptr = p
+
+    @staticmethod
+    def from_host(a: np.ndarray) -> "DevBuf":
+        a = np.ascontiguousarray(a)
+        b = DevBuf(max(a.nbytes, 4))
+        _check(lib().gpsat_memcpy_h2d(b.ptr, _p(a), ctypes.c_int64(a.nbytes)))
+        return b
+
+    def to_host(self, dtype, count: int) -> np.ndarray:
+        out = np.empty(count, dtype=dtype)
+        _check(lib().gpsat_memcpy_d2h(_p(out), self.ptr, ctypes.c_int64(out.nbytes)))
+        return out
+
+    def zero(self):
+        _check(lib().gpsat_memset_dev(self.ptr, 0, ctypes.c_int64(self.nbytes)))
+
+    def close(self):
+        if self.ptr:
+            lib().gpsat_dev_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        p = ctypes.c_void_p()
+        _check(lib().gpsat_event_create(ctypes.byref(p)))
+        self.ptr = p
+
+    def record(self, batch: Optional["GpuBatch"] = None):
+        _check(lib().gpsat_event_record(self.ptr, batch.h if batch is not None else None))
+
+    def elapsed_ms(self, stop: "Event") -> float:
+        ms = ctypes.c_float(0)
+        _check(lib().gpsat_event_elapsed_ms(self.ptr, stop.ptr, ctypes.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().gpsat_event_destroy(self.ptr)
+        except Exception:
+            pass
+
+
+class GpuBatch:
+    """A formula batch resident in HBM (gpsat_batch handle)."""
+
+    def __init__(self, batch: Batch):
+        self.batch = batch
+        self.h = ctypes.c_void_p()
+        self._keep = [np.ascontiguousarray(batch.lits, dtype=np.int32),
+                      np.ascontiguousarray(batch.clause_off, dtype=np.int64),
+                      np.ascontiguousarray(batch.inst_clause_off, dtype=np.int64),
+                      np.ascontiguousarray(batch.n_vars, dtype=np.int32)]
+        _check(lib().gpsat_batch_create(*(_p(a) for a in self._keep), ctypes.c_int32(batch.n_inst),
+                                        ctypes.byref(self.h)))
+
+    @property
+    def n_inst(self) -> int:
+        return self.batch.n_inst
+
+    def row_words(self, A: int) -> int:
+        w = ctypes.c_int64(0)
+        _check(lib().gpsat_batch_row_words(self.h, ctypes.c_int32(A), ctypes.byref(w)))
+        return w.value
+
+    def sync(self):
+        _check(lib().gpsat_batch_sync(self.h))
+
+    # ---- host-buffer forms
+    def clause_eval(self, rows: np.ndarray, A: int) -> np.ndarray:
+        rows = np.ascontiguousarray(rows, dtype=np.uint32)
+        assert rows.size == self.row_words(A)
+        counts = np.zeros((self.n_inst, A, 3), dtype=np.int32)
+        _check(lib().gpsat_clause_eval(self.h, _p(rows), ctypes.c_int32(A), _p(counts)))
+        return counts
+
+    def bcp_fixpoint(self, rows: np.ndarray, A: int):
+        rows = np.ascontiguousarray(rows, dtype=np.uint32)
+        assert rows.size == self.row_words(A)
+        out = np.zeros_like(rows)
+        conflict = np.zeros((self.n_inst, A), dtype=np.int32)
+        props = np.zeros((self.n_inst, A), dtype=np.int32)
+        _check(lib().gpsat_bcp_fixpoint(self.h, _p(rows), ctypes.c_int32(A), _p(out), _p(conflict), _p(props)))
+        return out, conflict, props
+
+    # ---- device-buffer forms (asynchronous on the batch stream)
+    def clause_eval_dev(self, rows: DevBuf, A: int, counts: DevBuf):
+        _check(lib().gpsat_clause_eval_dev(self.h, rows.ptr, ctypes.c_int32(A), counts.ptr))
+
+    def bcp_fixpoint_dev(self, rows: DevBuf, A: int, out: DevBuf, conflict: DevBuf, props: DevBuf):
+        _check(lib().gpsat_bcp_fixpoint_dev(self.h, rows.ptr, ctypes.c_int32(A), out.ptr, conflict.ptr, props.ptr))
+
+    def eval_bcp_dev(self, rows: DevBuf, A: int, counts: DevBuf, out: DevBuf, conflict: DevBuf, props: DevBuf):
+        _check(lib().gpsat_eval_bcp_dev(self.h, rows.ptr, ctypes.c_int32(A), counts.ptr, out.ptr, conflict.ptr,
+                                        props.ptr))
+
+    def close(self):
+        if self.h:
+            lib().gpsat_batch_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()

@@ -1,0 +1,345 @@
+// capi.cpp -- the extern "C" boundary of libgpsat.so: batches, clause evaluation,
+// BCP fixpoints and the device-memory / timing helpers.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+
+#include "../../include/gpsat.h"
+#include "gpsat_internal.hpp"
+
+namespace gpsat {
+static thread_local std::string g_last_error;
+void set_error(const std::string& msg) { g_last_error = msg; }
+}  // namespace gpsat
+
+using namespace gpsat;
+
+struct gpsat_batch {
+    DeviceBatch d;
+};
+
+#define HIP_TRY(expr)                                                                          \
+    do {                                                                                       \
+        hipError_t _e = (expr);                                                                \
+        if (_e != hipSuccess) {                                                                \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(_e));                      \
+            return GPSAT_ERR_HIP;                                                              \
+        }                                                                                      \
+    } while (0)
+
+#define ARG_CHECK(cond, msg)                 \
+    do {                                     \
+        if (!(cond)) {                       \
+            set_error(msg);                  \
+            return GPSAT_ERR_ARG;            \
+        }                                    \
+    } while (0)
+
+extern "C" {
+
+const char* gpsat_last_error(void) { return g_last_error.c_str(); }
+const char* gpsat_version(void) { return "v0.0.1"; }
+
+int gpsat_device_count(int32_t* count)
+{
+    ARG_CHECK(count, "count is null");
+    int c = 0;
+    if (hipGetDeviceCount(&c) != hipSuccess) c = 0;
+    *count = c;
+    return GPSAT_OK;
+}
+
+int gpsat_device_synchronize(void)
+{
+    HIP_TRY(hipDeviceSynchronize());
+    return GPSAT_OK;
+}
+
+int gpsat_dev_alloc(int64_t bytes, void** ptr)
+{
+    ARG_CHECK(ptr && bytes >= 0, "bad allocation request");
+    *ptr = nullptr;
+    if (bytes == 0) return GPSAT_OK;
+    hipError_t e = hipMalloc(ptr, (size_t)bytes);
+    if (e != hipSuccess) {
+        set_error(std::string("hipMalloc: ") + hipGetErrorString(e));
+        return e == hipErrorOutOfMemory ? GPSAT_ERR_NOMEM : GPSAT_ERR_HIP;
+    }
+    return GPSAT_OK;
+}
+
+int gpsat_dev_free(void* ptr)
+{
+    if (ptr) HIP_TRY(hipFree(ptr));
+    return GPSAT_OK;
+}
+
+int gpsat_memcpy_h2d(void* dst, const void* src, int64_t bytes)
+{
+    ARG_CHECK(bytes >= 0, "negative size");
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyHostToDevice));
+    return GPSAT_OK;
+}
+
+int gpsat_memcpy_d2h(void* dst, const void* src, int64_t bytes)
+{
+    ARG_CHECK(bytes >= 0, "negative size");
+    if (bytes) HIP_TRY(hipMemcpy(dst, src, (size_t)bytes, hipMemcpyDeviceToHost));
+    return GPSAT_OK;
+}
+
+int gpsat_memset_dev(void* dst, int32_t value, int64_t bytes)
+{
+    ARG_CHECK(bytes >= 0, "negative size");
+    if (bytes) HIP_TRY(hipMemset(dst, value, (size_t)bytes));
+    return GPSAT_OK;
+}
+
+int gpsat_event_create(void** ev)
+{
+    ARG_CHECK(ev, "ev is null");
+    hipEvent_t e;
+    HIP_TRY(hipEventCreate(&e));
+    *ev = (void*)e;
+    return GPSAT_OK;
+}
+
+int gpsat_event_destroy(void* ev)
+{
+    if (ev) HIP_TRY(hipEventDestroy((hipEvent_t)ev));
+    return GPSAT_OK;
+}
+
+int gpsat_event_record(void* ev, gpsat_batch* b)
+{
+    ARG_CHECK(ev, "ev is null");
+    HIP_TRY(hipEventRecord((hipEvent_t)ev, b ? b->d.stream : nullptr));
+    return GPSAT_OK;
+}
+
+int gpsat_event_elapsed_ms(void* start, void* stop, float* ms)
+{
+    ARG_CHECK(start && stop && ms, "null argument");
+    HIP_TRY(hipEventSynchronize((hipEvent_t)stop));
+    HIP_TRY(hipEventElapsedTime(ms, (hipEvent_t)start, (hipEvent_t)stop));
+    return GPSAT_OK;
+}
+
+int gpsat_copy_dev(gpsat_batch* b, void* dst, const void* src, int64_t bytes)
+{
+    ARG_CHECK(bytes >= 0 && bytes % 16 == 0, "copy size must be a multiple of 16");
+    ARG_CHECK(bytes == 0 || (dst && src), "null device buffer");
+    HIP_TRY(launch_copy(src, dst, bytes, b ? b->d.stream : nullptr));
+    return GPSAT_OK;
+}
+
+int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int64_t* inst_clause_off,
+                       const int32_t* n_vars, int32_t n_inst, gpsat_batch** out)
+{
+    ARG_CHECK(out, "out is null");
+    *out = nullptr;
+    ARG_CHECK(n_inst >= 0, "negative instance count");
+    ARG_CHECK(n_inst == 0 || (clause_off && inst_clause_off && n_vars), "null batch arrays");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no HIP device visible");
+        return GPSAT_ERR_NODEVICE;
+    }
+    gpsat_batch* b = new gpsat_batch();
+    DeviceBatch& d = b->d;
+    d.n_inst = n_inst;
+    const int64_t M = n_inst ? inst_clause_off[n_inst] - inst_clause_off[0] : 0;
+    const int64_t c0 = n_inst ? inst_clause_off[0] : 0;
+    const int64_t L = n_inst ? clause_off[c0 + M] - clause_off[c0] : 0;
+    const int64_t l0 = n_inst ? clause_off[c0] : 0;
+    d.n_clauses = M;
+    d.n_lits = L;
+    d.h_meta.resize(n_inst);
+    d.h_coff.resize(M + n_inst);
+    d.h_lits.assign(lits ? lits + l0 : nullptr, lits ? lits + l0 + L : nullptr);
+    int64_t wbase = 0;
+    for (int32_t i = 0; i < n_inst; i++) {
+        const int64_t cb = inst_clause_off[i], ce = inst_clause_off[i + 1];
+        if (ce < cb || n_vars[i] < 0) {
+            delete b;
+            set_error("malformed batch offsets");
+            return GPSAT_ERR_ARG;
+        }
+        InstMeta& mi = d.h_meta[i];
+        mi.lit_base = clause_off[cb] - l0;
+        mi.coff_base = (cb - c0) + i;
+        mi.word_base = wbase;
+        mi.m = (int32_t)(ce - cb);
+        mi.n = n_vars[i];
+        for (int64_t c = cb; c <= ce; c++) {
+            const int64_t rel = clause_off[c] - clause_off[cb];
+            if (rel < 0 || rel > INT32_MAX) {
+                delete b;
+                set_error("clause offsets out of range");
+                return GPSAT_ERR_ARG;
+            }
+            d.h_coff[mi.coff_base + (c - cb)] = (int32_t)rel;
+        }
+        for (int64_t k = clause_off[cb]; k < clause_off[ce]; k++) {
+            const int32_t l = lits[k];
+            if (l < 0 || (l >> 1) >= n_vars[i]) {
+                delete b;
+                set_error("literal code out of range for its instance");
+                return GPSAT_ERR_ARG;
+            }
+        }
+        wbase += words_per_row(mi.n);
+        d.max_n = std::max(d.max_n, mi.n);
+        d.max_m = std::max(d.max_m, mi.m);
+    }
+    d.total_words = wbase;
+    hipError_t e = hipGetDevice(&d.device);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
+    if (e == hipSuccess && n_inst) e = hipMalloc(&d.meta, sizeof(InstMeta) * n_inst);
+    if (e == hipSuccess && L) e = hipMalloc(&d.lits, sizeof(int32_t) * L);
+    if (e == hipSuccess && n_inst) e = hipMalloc(&d.coff, sizeof(int32_t) * (M + n_inst));
+    if (e == hipSuccess && n_inst) e = hipMemcpy(d.meta, d.h_meta.data(), sizeof(InstMeta) * n_inst, hipMemcpyHostToDevice);
+    if (e == hipSuccess && L) e = hipMemcpy(d.lits, d.h_lits.data(), sizeof(int32_t) * L, hipMemcpyHostToDevice);
+    if (e == hipSuccess && n_inst)
+        e = hipMemcpy(d.coff, d.h_coff.data(), sizeof(int32_t) * (M + n_inst), hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(std::string("batch upload: ") + hipGetErrorString(e));
+        gpsat_batch_destroy(b);
+        return e == hipErrorOutOfMemory ? GPSAT_ERR_NOMEM : GPSAT_ERR_HIP;
+    }
+    *out = b;
+    return GPSAT_OK;
+}
+
+int gpsat_batch_destroy(gpsat_batch* b)
+{
+    if (!b) return GPSAT_OK;
+    DeviceBatch& d = b->d;
+    if (d.stream) (void)hipStreamSynchronize(d.stream);
+    (void)hipFree(d.meta);
+    (void)hipFree(d.lits);
+    (void)hipFree(d.coff);
+    if (d.stream) (void)hipStreamDestroy(d.stream);
+    delete b;
+    return GPSAT_OK;
+}
+
+int gpsat_batch_info(const gpsat_batch* b, int32_t* n_inst, int64_t* n_clauses, int64_t* n_lits, int32_t* max_vars)
+{
+    ARG_CHECK(b, "null batch");
+    if (n_inst) *n_inst = b->d.n_inst;
+    if (n_clauses) *n_clauses = b->d.n_clauses;
+    if (n_lits) *n_lits = b->d.n_lits;
+    if (max_vars) *max_vars = b->d.max_n;
+    return GPSAT_OK;
+}
+
+int gpsat_batch_row_words(const gpsat_batch* b, int32_t A, int64_t* total_words)
+{
+    ARG_CHECK(b && total_words, "null argument");
+    ARG_CHECK(A >= 0, "negative A");
+    *total_words = (int64_t)A * b->d.total_words;
+    return GPSAT_OK;
+}
+
+int gpsat_batch_sync(gpsat_batch* b)
+{
+    ARG_CHECK(b, "null batch");
+    HIP_TRY(hipStreamSynchronize(b->d.stream));
+    return GPSAT_OK;
+}
+
+static int check_A(int32_t A)
+{
+    if (A < 1 || A > 64) {
+        set_error("A must be in [1, 64]");
+        return GPSAT_ERR_ARG;
+    }
+    return GPSAT_OK;
+}
+
+int gpsat_clause_eval_dev(gpsat_batch* b, const uint32_t* rows_dev, int32_t A, int32_t* counts_dev)
+{
+    ARG_CHECK(b, "null batch");
+    if (int r = check_A(A)) return r;
+    ARG_CHECK(b->d.n_inst == 0 || (rows_dev && counts_dev), "null device buffer");
+    HIP_TRY(launch_eval_bcp(b->d, rows_dev, A, counts_dev, nullptr, nullptr, nullptr, nullptr, true, false,
+                            b->d.stream));
+    return GPSAT_OK;
+}
+
+int gpsat_bcp_fixpoint_dev(gpsat_batch* b, const uint32_t* rows_dev, int32_t A, uint32_t* out_rows_dev,
+                           int32_t* conflict_dev, int32_t* props_dev)
+{
+    ARG_CHECK(b, "null batch");
+    if (int r = check_A(A)) return r;
+    ARG_CHECK(b->d.n_inst == 0 || (rows_dev && out_rows_dev && conflict_dev && props_dev), "null device buffer");
+    HIP_TRY(launch_eval_bcp(b->d, rows_dev, A, nullptr, out_rows_dev, conflict_dev, props_dev, nullptr, false, true,
+                            b->d.stream));
+    return GPSAT_OK;
+}
+
+int gpsat_eval_bcp_dev(gpsat_batch* b, const uint32_t* rows_dev, int32_t A, int32_t* counts_dev,
+                       uint32_t* out_rows_dev, int32_t* conflict_dev, int32_t* props_dev)
+{
+    ARG_CHECK(b, "null batch");
+    if (int r = check_A(A)) return r;
+    ARG_CHECK(b->d.n_inst == 0 || (rows_dev && counts_dev && out_rows_dev && conflict_dev && props_dev),
+              "null device buffer");
+    HIP_TRY(launch_eval_bcp(b->d, rows_dev, A, counts_dev, out_rows_dev, conflict_dev, props_dev, nullptr, true, true,
+                            b->d.stream));
+    return GPSAT_OK;
+}
+
+// Host-buffer forms: stage in HBM, run, copy back, free.
+namespace {
+struct DevBuf {
+    void* p = nullptr;
+    ~DevBuf() { (void)hipFree(p); }
+};
+}  // namespace
+
+int gpsat_clause_eval(gpsat_batch* b, const uint32_t* rows, int32_t A, int32_t* counts)
+{
+    ARG_CHECK(b, "null batch");
+    if (int r = check_A(A)) return r;
+    if (b->d.n_inst == 0) return GPSAT_OK;
+    ARG_CHECK(rows && counts, "null host buffer");
+    const size_t rw = (size_t)A * b->d.total_words * 4, cb = (size_t)b->d.n_inst * A * 3 * 4;
+    DevBuf dr, dc;
+    HIP_TRY(hipMalloc(&dr.p, std::max<size_t>(rw, 4)));
+    HIP_TRY(hipMalloc(&dc.p, cb));
+    HIP_TRY(hipMemcpyAsync(dr.p, rows, rw, hipMemcpyHostToDevice, b->d.stream));
+    HIP_TRY(launch_eval_bcp(b->d, (const uint32_t*)dr.p, A, (int32_t*)dc.p, nullptr, nullptr, nullptr, nullptr, true,
+                            false, b->d.stream));
+    HIP_TRY(hipMemcpyAsync(counts, dc.p, cb, hipMemcpyDeviceToHost, b->d.stream));
+    HIP_TRY(hipStreamSynchronize(b->d.stream));
+    return GPSAT_OK;
+}
+
+int gpsat_bcp_fixpoint(gpsat_batch* b, const uint32_t* rows, int32_t A, uint32_t* out_rows, int32_t* conflict,
+                       int32_t* props)
+{
+    ARG_CHECK(b, "null batch");
+    if (int r = check_A(A)) return r;
+    if (b->d.n_inst == 0) return GPSAT_OK;
+    ARG_CHECK(rows && out_rows && conflict && props, "null host buffer");
+    const size_t rw = (size_t)A * b->d.total_words * 4, fb = (size_t)b->d.n_inst * A * 4;
+    DevBuf dr, dout, dcf, dpr;
+    HIP_TRY(hipMalloc(&dr.p, std::max<size_t>(rw, 4)));
+    HIP_TRY(hipMalloc(&dout.p, std::max<size_t>(rw, 4)));
+    HIP_TRY(hipMalloc(&dcf.p, fb));
+    HIP_TRY(hipMalloc(&dpr.p, fb));
+    HIP_TRY(hipMemcpyAsync(dr.p, rows, rw, hipMemcpyHostToDevice, b->d.stream));
+    HIP_TRY(launch_eval_bcp(b->d, (const uint32_t*)dr.p, A, nullptr, (uint32_t*)dout.p, (int32_t*)dcf.p,
+                            (int32_t*)dpr.p, nullptr, false, true, b->d.stream));
+    HIP_TRY(hipMemcpyAsync(out_rows, dout.p, rw, hipMemcpyDeviceToHost, b->d.stream));
+    HIP_TRY(hipMemcpyAsync(conflict, dcf.p, fb, hipMemcpyDeviceToHost, b->d.stream));
+    HIP_TRY(hipMemcpyAsync(props, dpr.p, fb, hipMemcpyDeviceToHost, b->d.stream));
+    HIP_TRY(hipStreamSynchronize(b->d.stream));
+    return GPSAT_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,41 @@
+// gpsat_internal.hpp -- host-side structures shared by the C-ABI and the kernel launchers.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "common.hpp"
+
+namespace gpsat {
+
+// A batch of instances resident in HBM.
+struct DeviceBatch {
+    int32_t n_inst = 0;
+    int32_t max_n = 0;
+    int32_t max_m = 0;
+    int64_t n_clauses = 0;
+    int64_t n_lits = 0;
+    int64_t total_words = 0;      // sum of W_i (rows buffer holds A * total_words words)
+    InstMeta* meta = nullptr;     // device [n_inst]
+    int32_t* lits = nullptr;      // device [n_lits]
+    int32_t* coff = nullptr;      // device [n_clauses + n_inst], per-instance relative offsets
+    hipStream_t stream = nullptr;
+    int device = 0;
+    // host mirrors used by the solver set-up and the job chooser
+    std::vector<InstMeta> h_meta;
+    std::vector<int32_t> h_lits;
+    std::vector<int32_t> h_coff;
+};
+
+void set_error(const std::string& msg);
+
+size_t eval_bcp_lds_bytes(int max_n);
+hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, int32_t* counts, uint32_t* out_rows,
+                           int32_t* conflict, int32_t* props, int32_t* passes, bool do_eval, bool do_bcp,
+                           hipStream_t st);
+
+hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
+
+}  // namespace gpsat
