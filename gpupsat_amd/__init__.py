@@ -117,6 +117,42 @@ class Event:
             pass
 
 
+class SolveOpts(ctypes.Structure):
+    """Mirror of gpsat_solve_opts (include/gpsat.h)."""
+    _fields_ = [
+        ("decision", ctypes.c_int32), ("restart_first", ctypes.c_int32), ("restart_factor", ctypes.c_float),
+        ("learnt_cap", ctypes.c_int32), ("larena_cap", ctypes.c_int32), ("watch_slack", ctypes.c_int32),
+        ("max_conflicts", ctypes.c_int64), ("max_decisions", ctypes.c_int64), ("var_decay", ctypes.c_float),
+        ("phase_init", ctypes.c_int32), ("max_ms", ctypes.c_double), ("waves", ctypes.c_int32),
+        ("clause_mem", ctypes.c_int32), ("verify", ctypes.c_int32),
+    ]
+
+    @staticmethod
+    def default(**kw) -> "SolveOpts":
+        o = SolveOpts()
+        _check(lib().gpsat_solve_opts_default(ctypes.byref(o)))
+        for k, v in kw.items():
+            if not hasattr(o, k):
+                raise KeyError(k)
+            setattr(o, k, v)
+        return o
+
+
+class SolveInfo(ctypes.Structure):
+    _fields_ = [
+        ("kernel_ms", ctypes.c_double), ("verify_ms", ctypes.c_double), ("waves", ctypes.c_int32),
+        ("clause_mem", ctypes.c_int32), ("lds_bytes", ctypes.c_int64), ("workspace_bytes", ctypes.c_int64),
+        ("n_sat", ctypes.c_int32), ("n_unsat", ctypes.c_int32), ("n_undef", ctypes.c_int32),
+        ("n_verified", ctypes.c_int32), ("n_errors", ctypes.c_int32),
+    ]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+JOB_STAT_FIELDS = ("status", "conflicts", "decisions", "props", "learned", "restarts", "max_level", "reserved")
+
+
 class GpuBatch:
     """A formula batch resident in HBM (gpsat_batch handle)."""
 
@@ -169,6 +205,53 @@ class GpuBatch:
     def eval_bcp_dev(self, rows: DevBuf, A: int, counts: DevBuf, out: DevBuf, conflict: DevBuf, props: DevBuf):
         _check(lib().gpsat_eval_bcp_dev(self.h, rows.ptr, ctypes.c_int32(A), counts.ptr, out.ptr, conflict.ptr,
                                         props.ptr))
+
+    # ---- solve path
+    def solve(self, opts: Optional[SolveOpts] = None, jobs=None, act_init=None, phase_init=None,
+              want_models: bool = True, want_bumps: bool = False) -> dict:
+        """jobs: None (one job per instance) or (job_inst int32[J], cube_off int32[J+1], cube_lits int32[])."""
+        o = opts if opts is not None else SolveOpts.default()
+        N = self.n_inst
+        status = np.full(N, 2, dtype=np.int32)
+        verified = np.zeros(N, dtype=np.int32)
+        rows = np.zeros(max(self.row_words(1), 1), dtype=np.uint32)
+        if jobs is None:
+            ji = co = cl = None
+            J = N
+        else:
+            ji = np.ascontiguousarray(jobs[0], dtype=np.int32)
+            co = np.ascontiguousarray(jobs[1], dtype=np.int32)
+            cl = np.ascontiguousarray(jobs[2] if len(jobs[2]) else np.zeros(1, np.int32), dtype=np.int32)
+            J = len(ji)
+        js = np.zeros((max(J, 1), 8), dtype=np.int64)
+        nv = int(self.batch.n_vars.astype(np.int64).sum())
+        bumps = np.zeros(max(nv, 1), dtype=np.int32) if want_bumps else None
+        act = None if act_init is None else np.ascontiguousarray(act_init, dtype=np.float32)
+        ph = None if phase_init is None else np.ascontiguousarray(phase_init, dtype=np.int8)
+        info = SolveInfo()
+        P = lambda a: _p(a) if a is not None else None
+        rc = lib().gpsat_solve(self.h, ctypes.byref(o), P(ji), P(co), P(cl), ctypes.c_int32(J), P(act), P(ph),
+                               _p(status), _p(rows), _p(verified), _p(js), P(bumps), ctypes.byref(info))
+        _check(rc)
+        out = {"status": status, "verified": verified, "job_stats": js[:J], "info": info.as_dict()}
+        if want_models:
+            out["model_rows"] = rows
+        if want_bumps:
+            out["var_bumps"] = bumps[:nv]
+        return out
+
+    def models(self, model_rows: np.ndarray) -> list:
+        """Split solve() model rows into int8[n] models (1 true, 0 false, -1 unassigned)."""
+        base = self.batch.row_base(1)
+        out = []
+        for i in range(self.n_inst):
+            n = int(self.batch.n_vars[i])
+            t, f = unpack_rows(model_rows[base[i]:base[i + 1]][None, :], n)
+            m = np.full(n, -1, dtype=np.int8)
+            m[t[0]] = 1
+            m[f[0]] = 0
+            out.append(m)
+        return out
 
     def close(self):
         if self.h:

@@ -16,10 +16,6 @@ void set_error(const std::string& msg) { g_last_error = msg; }
 
 using namespace gpsat;
 
-struct gpsat_batch {
-    DeviceBatch d;
-};
-
 #define HIP_TRY(expr)                                                                          \
     do {                                                                                       \
         hipError_t _e = (expr);                                                                \

@@ -39,3 +39,8 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
 
 }  // namespace gpsat
+
+// The opaque handle behind the C ABI's gpsat_batch.
+struct gpsat_batch {
+    gpsat::DeviceBatch d;
+};

@@ -1,0 +1,824 @@
+// solver.hip -- CDCL on the device: one wavefront solves one job (an instance, or an
+// instance plus a cube of assumed literals); a persistent grid pulls jobs from a
+// device work queue until it is empty.
+//
+// Reference path restated in wave-parallel form (src/SATSolver/SATSolver.cu:67-218):
+//   decide -> propagate -> on conflict analyse, learn, backjump -> restart.
+// Strategy surfaces of the reference and where they live here:
+//   BCPStrategy       (src/BCPStrategy/WatchedClausesList.cu:46-309): two watched literals,
+//                     per-literal watch lists; a wave walks a list 64 entries at a time
+//                     and finds units / conflicts with ballots.
+//   ConflictAnalysis  (src/ConflictAnalysis/GraphAnalyzer.cu:31-138): 1-UIP over the trail
+//                     (the reference's decision-scheme analysis is kept in the CPU oracle).
+//   ClauseLearning    (src/ClauseLearning/LearntClausesManager.cu:16-41): learned clauses are
+//                     appended to a per-job arena and watched; the DB is reduced at restarts.
+//   DecisionStrategy  (src/DecisionStrategy/VSIDS.cu:77-140, src/SATSolver/DecisionMaker.cu:45-55):
+//                     VSIDS with phase saving, or the reference's last-free-var/positive rule.
+//   Restarts          (src/Restarts/GeometricRestartsManager.cu:3-31): geometric schedule.
+//   JobsQueue         (src/SATSolver/JobsQueue.cu:10-32): atomic queue head; every cube of an
+//                     instance must be refuted before the instance is UNSAT.
+#include "common.hpp"
+#include "gpsat_internal.hpp"
+#include "solver.hpp"
+
+namespace gpsat {
+
+namespace {
+
+constexpr int32_t F_DELETED = 1;
+constexpr int32_t F_USED = 2;
+constexpr int32_t F_LOCKED = 4;
+
+__device__ __forceinline__ int32_t prefix_count(uint64_t m)
+{
+    return (int32_t)__builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+__device__ __forceinline__ int32_t popc64(uint64_t m) { return __builtin_popcountll(m); }
+__device__ __forceinline__ int32_t ctz64(uint64_t m) { return __builtin_ctzll(m); }
+
+__device__ __forceinline__ int32_t wave_inclusive_scan(int32_t x)
+{
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        int32_t y = __shfl_up(x, s);
+        if (lane_id() >= s) x += y;
+    }
+    return x;
+}
+
+// Max over the wave of a 64-bit key.
+__device__ __forceinline__ uint64_t wave_max_u64(uint64_t k)
+{
+#pragma unroll
+    for (int s = 1; s < 64; s <<= 1) {
+        uint32_t lo = __shfl_xor((uint32_t)k, s);
+        uint32_t hi = __shfl_xor((uint32_t)(k >> 32), s);
+        uint64_t o = ((uint64_t)hi << 32) | lo;
+        k = o > k ? o : k;
+    }
+    return k;
+}
+
+__device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+struct Ctx {
+    // instance (uniform)
+    int32_t n, m, inst;
+    const int32_t* lits;
+    const int32_t* coff;
+    // per-var / per-literal state (LDS)
+    int8_t* lv;          // [2n] literal value: 1 true, -1 false, 0 unassigned
+    int8_t* phase;       // [n]
+    int32_t* seen;       // [n]
+    int32_t* level;      // [n]
+    int32_t* reason;     // [n]  clause id, -1 for decisions and level-0 facts
+    int32_t* trail;      // [n]
+    int32_t* tlim;       // [n+2] trail index where each level starts
+    float* act;          // [n]
+    int32_t* wstart;     // [2n+1] start of each original-clause watch list in Wo
+    int32_t* wcnt_o;     // [2n]
+    int32_t* wcnt_l;     // [2n]
+    int32_t* lbuf;       // [n+1] clause being learned
+    int32_t* lstamp;     // [n+2] level stamps for LBD
+    int32_t* bumps;      // [n]
+    // clause part (LDS or HBM)
+    int32_t* Wo;         // original-clause watch lists (capacity = occurrences)
+    int32_t* Wl;         // learned-clause watch lists, slack slots per literal
+    int32_t* wpair;      // [2*(m + learnt_cap)] the two watched literals of every clause
+    int32_t* lmeta;      // [4*learnt_cap] start, len, flags, lbd; then [learnt_cap] remap slots
+    int32_t* larena;     // learned literals
+    int32_t slack, learnt_cap, larena_cap;
+    // search state (uniform)
+    int32_t dl, tsize, qhead;
+    int32_t nlearnt, larena_used;
+    float inc;
+    int32_t stamp;
+    int64_t conflicts, decisions, props, learned, restarts;
+    int32_t levelmax;
+};
+
+__device__ __forceinline__ void clause_span(const Ctx& J, int32_t c, const int32_t*& cl, int32_t& len)
+{
+    if (c < J.m) {
+        const int32_t b = J.coff[c];
+        len = J.coff[c + 1] - b;
+        cl = J.lits + b;
+    } else {
+        const int32_t* mt = J.lmeta + 4 * (c - J.m);
+        cl = J.larena + mt[0];
+        len = mt[1];
+    }
+}
+
+// Uniform assignment: every lane stores the same values.
+__device__ __forceinline__ void assign(Ctx& J, int32_t u, int32_t r)
+{
+    const int32_t v = u >> 1;
+    J.lv[u] = 1;
+    J.lv[u ^ 1] = -1;
+    J.level[v] = J.dl;
+    J.reason[v] = r;
+    J.trail[J.tsize] = u;
+    J.tsize++;
+}
+
+// Walk the watch list of the false literal fl, 64 entries per step.  Moves watches,
+// compacts the list in place, assigns units in lane order and reports the first
+// conflict.  Returns the new list length.
+template <bool LEARNT>
+__device__ int32_t walk_list(Ctx& J, int32_t fl, int32_t* W, int32_t base, int32_t cnt, int32_t& confl)
+{
+    const int lane = lane_id();
+    int32_t keep = 0;
+    for (int32_t i0 = 0; i0 < cnt; i0 += kWave) {
+        const int32_t i = i0 + lane;
+        const bool active = i < cnt;
+        const int32_t c = active ? W[base + i] : 0;
+        bool kept = active;
+        int32_t unit = -1;
+        bool conf = false;
+        if (active && confl < 0) {
+            bool dead = false;
+            if (LEARNT) dead = (J.lmeta[4 * (c - J.m) + 2] & F_DELETED) != 0;
+            if (dead) {
+                kept = false;
+            } else {
+                const int32_t w0 = J.wpair[2 * c], w1 = J.wpair[2 * c + 1];
+                const int32_t other = (w0 == fl) ? w1 : w0;
+                if (w0 != fl && w1 != fl) {
+                    kept = false;   // stale entry: the clause moved its watch away from fl
+                } else if (J.lv[other] != 1) {
+                    const int32_t* cl;
+                    int32_t len;
+                    clause_span(J, c, cl, len);
+                    int32_t rep = -1;
+                    for (int32_t k = 0; k < len; k++) {
+                        const int32_t q = cl[k];
+                        if (q != w0 && q != w1 && J.lv[q] != -1) {
+                            rep = q;
+                            break;
+                        }
+                    }
+                    if (rep >= 0) {
+                        kept = false;
+                        if (!LEARNT) {
+                            const int32_t pos = atomicAdd(&J.wcnt_o[rep], 1);
+                            J.Wo[J.wstart[rep] + pos] = c;
+                            J.wpair[2 * c] = other;
+                            J.wpair[2 * c + 1] = rep;
+                        } else {
+                            const int32_t pos = atomicAdd(&J.wcnt_l[rep], 1);
+                            if (pos < J.slack) {
+                                J.Wl[rep * J.slack + pos] = c;
+                                J.wpair[2 * c] = other;
+                                J.wpair[2 * c + 1] = rep;
+                            } else {
+                                // no room: the learned clause stops propagating (it may still
+                                // be a reason; its literals stay in the arena)
+                                atomicSub(&J.wcnt_l[rep], 1);
+                                atomicOr(&J.lmeta[4 * (c - J.m) + 2], F_DELETED);
+                            }
+                        }
+                    } else if (J.lv[other] == -1) {
+                        conf = true;
+                    } else {
+                        unit = other;
+                    }
+                }
+            }
+        }
+        const uint64_t km = ballot(kept);
+        if (kept) W[base + keep + prefix_count(km)] = c;
+        keep += popc64(km);
+        const uint64_t cm = ballot(conf);
+        if (cm && confl < 0) confl = uni(readlane(c, ctz64(cm)));
+        uint64_t um = ballot(unit >= 0);
+        while (um && confl < 0) {
+            const int j = ctz64(um);
+            um &= um - 1;
+            const int32_t u = uni(readlane(unit, j));
+            const int32_t rc = uni(readlane(c, j));
+            const int8_t val = J.lv[u];
+            if (val == -1) confl = rc;
+            else if (val == 0) assign(J, u, rc);
+        }
+    }
+    return keep;
+}
+
+__device__ int32_t propagate(Ctx& J)
+{
+    int32_t confl = -1;
+    while (confl < 0 && J.qhead < J.tsize) {
+        const int32_t p = uni(J.trail[J.qhead]);
+        J.qhead++;
+        J.props++;
+        const int32_t fl = p ^ 1;
+        int32_t k = walk_list<false>(J, fl, J.Wo, uni(J.wstart[fl]), uni(J.wcnt_o[fl]), confl);
+        J.wcnt_o[fl] = k;
+        k = walk_list<true>(J, fl, J.Wl, fl * J.slack, uni(J.wcnt_l[fl]), confl);
+        J.wcnt_l[fl] = k;
+    }
+    return confl;
+}
+
+__device__ void backtrack(Ctx& J, int32_t b, bool save_phase)
+{
+    if (J.dl <= b) return;
+    const int32_t start = uni(J.tlim[b + 1]);
+    for (int32_t t = start + lane_id(); t < J.tsize; t += kWave) {
+        const int32_t u = J.trail[t];
+        const int32_t v = u >> 1;
+        J.lv[u] = 0;
+        J.lv[u ^ 1] = 0;
+        J.reason[v] = -1;
+        if (save_phase) J.phase[v] = (int8_t)(u & 1);
+    }
+    __syncthreads();
+    J.tsize = start;
+    J.qhead = start;
+    J.dl = b;
+}
+
+// 1-UIP.  Leaves the learned clause in lbuf[0..len) with the asserting literal first and
+// a literal of the backjump level second.
+__device__ int32_t analyze(Ctx& J, int32_t confl, int32_t& bt, int32_t& lbd)
+{
+    const int lane = lane_id();
+    int32_t nl = 1, pathc = 0, p = -1, idx = J.tsize - 1, c = confl;
+    for (;;) {
+        if (c >= J.m) {
+            const int32_t f = J.lmeta[4 * (c - J.m) + 2];
+            J.lmeta[4 * (c - J.m) + 2] = f | F_USED;
+        }
+        const int32_t* cl;
+        int32_t len;
+        clause_span(J, c, cl, len);
+        for (int32_t k0 = 0; k0 < len; k0 += kWave) {
+            const int32_t k = k0 + lane;
+            const int32_t q = k < len ? cl[k] : -1;
+            bool mark = false, cur = false;
+            if (q >= 0 && q != p) {
+                const int32_t v = q >> 1;
+                if (J.level[v] > 0 && atomicOr(&J.seen[v], 1) == 0) {
+                    mark = true;
+                    cur = J.level[v] == J.dl;
+                    J.act[v] += J.inc;
+                    J.bumps[v] += 1;
+                }
+            }
+            const uint64_t cmask = ballot(mark && cur);
+            pathc += popc64(cmask);
+            const uint64_t lmask = ballot(mark && !cur);
+            if (mark && !cur) J.lbuf[nl + prefix_count(lmask)] = q;
+            nl += popc64(lmask);
+        }
+        // next marked literal on the trail, walking back from idx
+        bool found = false;
+        while (idx >= 0) {
+            const int32_t t = idx - lane;
+            const bool hit = t >= 0 && J.seen[J.trail[t] >> 1] != 0;
+            const uint64_t hm = ballot(hit);
+            if (hm) {
+                idx -= ctz64(hm);
+                found = true;
+                break;
+            }
+            idx -= kWave;
+        }
+        if (!found) return -1;   // no current-level literal: internal error, reported by the caller
+        p = uni(J.trail[idx]);
+        idx--;
+        J.seen[p >> 1] = 0;
+        pathc--;
+        if (pathc <= 0) break;
+        c = uni(J.reason[p >> 1]);
+    }
+    J.lbuf[0] = p ^ 1;
+    __syncthreads();
+    // backjump level = highest level among the other literals; LBD = distinct levels
+    J.stamp++;
+    uint64_t best = 0;
+    int32_t distinct = 0;
+    for (int32_t k = 1 + lane; k < nl; k += kWave) {
+        const int32_t q = J.lbuf[k];
+        const int32_t v = q >> 1;
+        J.seen[v] = 0;
+        const int32_t lv = J.level[v];
+        const uint64_t key = ((uint64_t)(uint32_t)lv << 32) | (uint32_t)(0x7fffffff - k);
+        best = key > best ? key : best;
+        if (atomicExch(&J.lstamp[lv], J.stamp) != J.stamp) distinct++;
+    }
+    best = wave_max_u64(best);
+    for (int s = 32; s > 0; s >>= 1) distinct += __shfl_xor(distinct, s);
+    lbd = uni(distinct) + 1;
+    if (nl > 1) {
+        bt = uni((int32_t)(best >> 32));
+        const int32_t bi = uni(0x7fffffff - (int32_t)(uint32_t)best);
+        const int32_t t1 = J.lbuf[1], tb = J.lbuf[bi];
+        __syncthreads();
+        J.lbuf[1] = tb;
+        J.lbuf[bi] = t1;
+    } else {
+        bt = 0;
+    }
+    __syncthreads();
+    return nl;
+}
+
+// Append lbuf[0..len) as a learned clause.  Returns its clause id.
+__device__ int32_t store_learned(Ctx& J, int32_t len, int32_t lbd)
+{
+    const int32_t id = J.nlearnt;
+    const int32_t start = J.larena_used;
+    for (int32_t k = lane_id(); k < len; k += kWave) J.larena[start + k] = J.lbuf[k];
+    J.lmeta[4 * id + 0] = start;
+    J.lmeta[4 * id + 1] = len;
+    J.lmeta[4 * id + 2] = 0;
+    J.lmeta[4 * id + 3] = lbd;
+    J.nlearnt++;
+    J.larena_used += len;
+    J.learned++;
+    return J.m + id;
+}
+
+// Watch a learned clause on literals a and b (uniform).
+__device__ void watch_learned(Ctx& J, int32_t c, int32_t a, int32_t b)
+{
+    J.wpair[2 * c] = a;
+    J.wpair[2 * c + 1] = b;
+    const int32_t pa = uni(J.wcnt_l[a]);
+    const int32_t pb = uni(J.wcnt_l[b]) + (a == b ? 1 : 0);
+    if (pa < J.slack && pb < J.slack) {
+        J.Wl[a * J.slack + pa] = c;
+        J.wcnt_l[a] = pa + 1;
+        const int32_t pb2 = uni(J.wcnt_l[b]);
+        J.Wl[b * J.slack + pb2] = c;
+        J.wcnt_l[b] = pb2 + 1;
+    } else {
+        J.lmeta[4 * (c - J.m) + 2] |= F_DELETED;
+    }
+}
+
+// At level 0: attach learned clause i (an index into lmeta).  A clause satisfied at
+// level 0 is dropped, a clause with one non-false literal asserts it, a falsified
+// clause means the job is UNSAT (returns false).
+__device__ bool attach_level0(Ctx& J, int32_t i)
+{
+    const int lane = lane_id();
+    const int32_t st = uni(J.lmeta[4 * i]);
+    const int32_t len = uni(J.lmeta[4 * i + 1]);
+    int32_t nf[2] = {-1, -1};
+    int32_t nnf = 0;
+    bool sat = false;
+    for (int32_t k0 = 0; k0 < len && nnf < 2 && !sat; k0 += kWave) {
+        const int32_t k = k0 + lane;
+        const int32_t q = k < len ? J.larena[st + k] : -1;
+        const int8_t v = q >= 0 ? J.lv[q] : (int8_t)-1;
+        if (ballot(v == 1)) sat = true;
+        uint64_t um = ballot(v == 0);
+        while (um && nnf < 2) {
+            const int j = ctz64(um);
+            um &= um - 1;
+            nf[nnf++] = uni(readlane(q, j));
+        }
+    }
+    bool ok = true;
+    if (sat) {
+        J.lmeta[4 * i + 2] = F_DELETED;
+    } else if (nnf == 0) {
+        ok = false;
+    } else if (nnf == 1) {
+        J.lmeta[4 * i + 2] = F_DELETED;
+        if (J.lv[nf[0]] == 0) assign(J, nf[0], -1);
+    } else {
+        watch_learned(J, J.m + i, nf[0], nf[1]);
+    }
+    __syncthreads();
+    return ok;
+}
+
+// Reduce the learned DB at the current level.  Survivors: reasons of assigned literals
+// (always), glue clauses (LBD <= 3) and, in mode 0, clauses used since the last
+// reduction; mode 2 keeps the reasons only.  Clause ids are renumbered, reasons are
+// remapped and the learned watch lists are rebuilt from the watch pairs, which keeps
+// the two-watched-literal invariant at any level.
+__device__ void reduce_db(Ctx& J, int mode)
+{
+    const int lane = lane_id();
+    const int32_t m = J.m;
+    int32_t* remap = J.lmeta + 4 * J.learnt_cap;
+    for (int32_t t = lane; t < J.tsize; t += kWave) {
+        const int32_t v = J.trail[t] >> 1;
+        const int32_t r = J.reason[v];
+        if (r >= m && J.level[v] > 0) atomicOr(&J.lmeta[4 * (r - m) + 2], F_LOCKED);   // level-0 reasons are never read
+    }
+    __syncthreads();
+    int32_t ns = 0;
+    for (int32_t i0 = 0; i0 < J.nlearnt; i0 += kWave) {
+        const int32_t i = i0 + lane;
+        bool keep = false, del = false;
+        int32_t st = 0, len = 0, lbd = 0, w0 = 0, w1 = 0;
+        if (i < J.nlearnt) {
+            st = J.lmeta[4 * i];
+            len = J.lmeta[4 * i + 1];
+            const int32_t fl = J.lmeta[4 * i + 2];
+            lbd = J.lmeta[4 * i + 3];
+            w0 = J.wpair[2 * (m + i)];
+            w1 = J.wpair[2 * (m + i) + 1];
+            del = (fl & F_DELETED) != 0;
+            const bool wanted = mode == 0 ? (lbd <= 3 || (fl & F_USED)) : mode == 1 ? lbd <= 3 : false;
+            keep = (fl & F_LOCKED) || (!del && wanted);
+        }
+        const uint64_t km = ballot(keep);
+        __syncthreads();
+        const int32_t ni = ns + prefix_count(km);
+        if (i < J.nlearnt) remap[i] = keep ? ni : -1;
+        if (keep) {
+            J.lmeta[4 * ni] = st;
+            J.lmeta[4 * ni + 1] = len;
+            J.lmeta[4 * ni + 2] = del ? F_DELETED : 0;
+            J.lmeta[4 * ni + 3] = lbd;
+            J.wpair[2 * (m + ni)] = w0;
+            J.wpair[2 * (m + ni) + 1] = w1;
+        }
+        ns += popc64(km);
+        __syncthreads();
+    }
+    // compact literals (forward copy; a destination never lies above its source)
+    int32_t pos = 0;
+    for (int32_t i = 0; i < ns; i++) {
+        const int32_t st = uni(J.lmeta[4 * i]);
+        const int32_t len = uni(J.lmeta[4 * i + 1]);
+        for (int32_t k0 = 0; k0 < len; k0 += kWave) {
+            const int32_t k = k0 + lane;
+            const int32_t x = k < len ? J.larena[st + k] : 0;
+            __syncthreads();
+            if (k < len) J.larena[pos + k] = x;
+            __syncthreads();
+        }
+        J.lmeta[4 * i] = pos;
+        pos += len;
+    }
+    for (int32_t t = lane; t < J.tsize; t += kWave) {
+        const int32_t v = J.trail[t] >> 1;
+        const int32_t r = J.reason[v];
+        if (r >= m) J.reason[v] = remap[r - m] >= 0 ? m + remap[r - m] : -1;
+    }
+    for (int32_t l = lane; l < 2 * J.n; l += kWave) J.wcnt_l[l] = 0;
+    __syncthreads();
+    for (int32_t i = lane; i < ns; i += kWave) {
+        if (J.lmeta[4 * i + 2] & F_DELETED) continue;
+        const int32_t c = m + i;
+        const int32_t wl[2] = {J.wpair[2 * c], J.wpair[2 * c + 1]};
+        for (int h = 0; h < 2; h++) {
+            const int32_t p = atomicAdd(&J.wcnt_l[wl[h]], 1);
+            if (p < J.slack) {
+                J.Wl[wl[h] * J.slack + p] = c;
+            } else {
+                atomicSub(&J.wcnt_l[wl[h]], 1);
+                J.lmeta[4 * i + 2] = F_DELETED;
+            }
+        }
+    }
+    J.nlearnt = ns;
+    J.larena_used = pos;
+    __syncthreads();
+}
+
+__device__ int32_t pick_branch(Ctx& J, int mode)
+{
+    uint64_t best = 0;
+    for (int32_t v = lane_id(); v < J.n; v += kWave) {
+        if (J.lv[2 * v] == 0) {
+            uint32_t hi = mode == 1 ? 1u : (__float_as_uint(J.act[v]) | 0x80000000u) ;
+            // larger key wins: activity first (non-negative floats order as integers),
+            // then the lower var (VSIDS) or the higher var (reference: last free var)
+            uint32_t lo = mode == 1 ? (uint32_t)v + 1u : 0xffffffffu - (uint32_t)v;
+            uint64_t key = ((uint64_t)hi << 32) | lo;
+            best = key > best ? key : best;
+        }
+    }
+    best = wave_max_u64(best);
+    if (best == 0) return -1;
+    const uint32_t lo = (uint32_t)best;
+    return uni(mode == 1 ? (int32_t)(lo - 1u) : (int32_t)(0xffffffffu - lo));
+}
+
+__device__ void rescale_activity(Ctx& J)
+{
+    for (int32_t v = lane_id(); v < J.n; v += kWave) J.act[v] *= 1e-30f;
+    J.inc *= 1e-30f;
+    __syncthreads();
+}
+
+__device__ __forceinline__ int32_t load_status(int32_t* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Lay out the per-wave arrays.  The state part always sits in LDS; the clause part
+// sits in LDS (kLds) or in this wave's slice of the HBM workspace.
+struct Layout {
+    static __host__ __device__ size_t state_bytes(int32_t n)
+    {
+        size_t b = 0;
+        b += ((2 * (size_t)n + 3) & ~(size_t)3);        // lv
+        b += (((size_t)n + 3) & ~(size_t)3);            // phase
+        b += 4 * (size_t)n * 5;                         // seen level reason trail act
+        b += 4 * ((size_t)n + 2);                       // tlim
+        b += 4 * (2 * (size_t)n + 1);                   // wstart
+        b += 4 * (2 * (size_t)n) * 2;                   // wcnt_o wcnt_l
+        b += 4 * ((size_t)n + 1);                       // lbuf
+        b += 4 * ((size_t)n + 2);                       // lstamp
+        b += 4 * (size_t)n;                             // bumps
+        return (b + 15) & ~(size_t)15;
+    }
+};
+
+__device__ void carve_state(Ctx& J, char* s, int32_t n)
+{
+    J.lv = (int8_t*)s;                 s += ((2 * n + 3) & ~3);
+    J.phase = (int8_t*)s;              s += ((n + 3) & ~3);
+    J.seen = (int32_t*)s;              s += 4 * n;
+    J.level = (int32_t*)s;             s += 4 * n;
+    J.reason = (int32_t*)s;            s += 4 * n;
+    J.trail = (int32_t*)s;             s += 4 * n;
+    J.act = (float*)s;                 s += 4 * n;
+    J.tlim = (int32_t*)s;              s += 4 * (n + 2);
+    J.wstart = (int32_t*)s;            s += 4 * (2 * n + 1);
+    J.wcnt_o = (int32_t*)s;            s += 4 * (2 * n);
+    J.wcnt_l = (int32_t*)s;            s += 4 * (2 * n);
+    J.lbuf = (int32_t*)s;              s += 4 * (n + 1);
+    J.lstamp = (int32_t*)s;            s += 4 * (n + 2);
+    J.bumps = (int32_t*)s;
+}
+
+__device__ void carve_clauses(Ctx& J, int32_t* c, const SolveArgs& a)
+{
+    J.Wo = c;      c += a.wo_cap;
+    J.Wl = c;      c += a.wl_cap;
+    J.wpair = c;   c += a.wpair_cap;
+    J.lmeta = c;   c += a.lmeta_cap;
+    J.larena = c;
+}
+
+// Returns the job verdict: 0 SAT, 1 UNSAT, 2 UNDEF (budget), 3 skipped (instance decided).
+__device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd)
+{
+    const int lane = lane_id();
+    const SolveOptsDev& o = a.o;
+    const InstMeta mi = a.meta[jd.inst];
+    J.inst = jd.inst;
+    J.n = uni(mi.n);
+    J.m = uni(mi.m);
+    J.lits = a.lits + mi.lit_base;
+    J.coff = a.coff + mi.coff_base;
+    const int32_t n = J.n, m = J.m, n2 = 2 * n;
+    const int64_t vb = a.var_base[jd.inst];
+    J.slack = o.watch_slack;
+    J.learnt_cap = o.learnt_cap;
+    J.larena_cap = o.larena_cap;
+    J.dl = 0;
+    J.tsize = 0;
+    J.qhead = 0;
+    J.nlearnt = 0;
+    J.larena_used = 0;
+    J.inc = 1.0f;
+    J.stamp = 0;
+    J.conflicts = J.decisions = J.props = J.learned = J.restarts = 0;
+    J.levelmax = 0;
+
+    if (load_status(&a.inst_status[jd.inst]) != 2) return 3;
+    const uint64_t t_start = wall_clock64();
+
+    for (int32_t v = lane; v < n; v += kWave) {
+        J.seen[v] = 0;
+        J.level[v] = 0;
+        J.reason[v] = -1;
+        J.act[v] = a.act_init ? a.act_init[vb + v] : 0.0f;
+        J.phase[v] = a.phase_init ? a.phase_init[vb + v] : (int8_t)o.phase_init;
+        J.bumps[v] = 0;
+    }
+    for (int32_t v = lane; v < n + 2; v += kWave) J.lstamp[v] = 0;
+    for (int32_t l = lane; l < n2; l += kWave) {
+        J.lv[l] = 0;
+        J.wcnt_o[l] = 0;
+        J.wcnt_l[l] = 0;
+    }
+    __syncthreads();
+    // occurrence counts give the capacity of every original-clause watch list
+    const int32_t L = uni(J.coff[m]);
+    for (int32_t k = lane; k < L; k += kWave) atomicAdd(&J.wcnt_o[J.lits[k]], 1);
+    __syncthreads();
+    int32_t carry = 0;
+    for (int32_t l0 = 0; l0 < n2; l0 += kWave) {
+        const int32_t l = l0 + lane;
+        const int32_t x = l < n2 ? J.wcnt_o[l] : 0;
+        const int32_t inc = wave_inclusive_scan(x);
+        if (l < n2) {
+            J.wstart[l] = carry + inc - x;
+            J.wcnt_o[l] = 0;
+        }
+        carry += uni(__shfl(inc, 63));
+    }
+    if (lane == 0) J.wstart[n2] = carry;
+    __syncthreads();
+    // watch the first two literals of every clause; units and empty clauses at level 0
+    bool unsat = false;
+    for (int32_t c0 = 0; c0 < m && !unsat; c0 += kWave) {
+        const int32_t c = c0 + lane;
+        int32_t len = -1, l0 = -1;
+        if (c < m) {
+            const int32_t b = J.coff[c];
+            len = J.coff[c + 1] - b;
+            if (len >= 1) {
+                l0 = J.lits[b];
+                // the two watches are distinct literals; a clause repeating one literal is a unit
+                int32_t l1 = -1;
+                for (int32_t k = 1; k < len; k++) {
+                    const int32_t q = J.lits[b + k];
+                    if (q != l0) {
+                        l1 = q;
+                        break;
+                    }
+                }
+                if (l1 >= 0) {
+                    len = 2;
+                    J.wpair[2 * c] = l0;
+                    J.wpair[2 * c + 1] = l1;
+                    J.Wo[J.wstart[l0] + atomicAdd(&J.wcnt_o[l0], 1)] = c;
+                    J.Wo[J.wstart[l1] + atomicAdd(&J.wcnt_o[l1], 1)] = c;
+                } else {
+                    len = 1;
+                }
+            }
+        }
+        if (ballot(len == 0)) unsat = true;
+        uint64_t um = ballot(len == 1);
+        while (um && !unsat) {
+            const int j = ctz64(um);
+            um &= um - 1;
+            const int32_t u = uni(readlane(l0, j));
+            const int8_t val = J.lv[u];
+            if (val == -1) unsat = true;
+            else if (val == 0) assign(J, u, -1);
+        }
+    }
+    // the cube's literals are facts of this job
+    for (int32_t k = 0; k < jd.cube_len && !unsat; k++) {
+        const int32_t u = uni(a.cube_lits[jd.cube_off + k]);
+        const int8_t val = J.lv[u];
+        if (val == -1) unsat = true;
+        else if (val == 0) assign(J, u, -1);
+    }
+    __syncthreads();
+    if (unsat) return 1;
+    if (propagate(J) >= 0) return 1;
+
+    int32_t restart_thr = o.restart_first, restart_cnt = 0;
+    const float decay_inv = 1.0f / o.var_decay;
+    for (;;) {
+        const int32_t confl = propagate(J);
+        if (confl >= 0) {
+            J.conflicts++;
+            restart_cnt++;
+            if (J.dl == 0) return 1;
+            int32_t bt, lbd;
+            const int32_t nl = analyze(J, confl, bt, lbd);
+            if (nl < 0) return 4;
+            if (nl == 1) {
+                backtrack(J, 0, o.decision_mode == 0);
+                assign(J, uni(J.lbuf[0]), -1);
+            } else {
+                backtrack(J, bt, o.decision_mode == 0);
+                if (!(J.nlearnt < J.learnt_cap && J.larena_used + nl <= J.larena_cap)) {
+                    reduce_db(J, 0);
+                    if (!(J.nlearnt < J.learnt_cap && J.larena_used + nl <= J.larena_cap)) reduce_db(J, 2);
+                }
+                if (J.nlearnt < J.learnt_cap && J.larena_used + nl <= J.larena_cap) {
+                    const int32_t cid = store_learned(J, nl, lbd);
+                    __syncthreads();
+                    watch_learned(J, cid, uni(J.lbuf[0]), uni(J.lbuf[1]));
+                    assign(J, uni(J.lbuf[0]), cid);
+                } else {
+                    // every held clause is a reason: restart, then add the clause at level 0
+                    backtrack(J, 0, o.decision_mode == 0);
+                    J.restarts++;
+                    restart_cnt = 0;
+                    reduce_db(J, 2);
+                    const int32_t cid = store_learned(J, nl, lbd);
+                    __syncthreads();
+                    if (!attach_level0(J, cid - m)) return 1;
+                }
+            }
+            __syncthreads();
+            J.inc *= decay_inv;
+            if (J.inc > 1e30f) rescale_activity(J);
+            if (o.max_conflicts > 0 && J.conflicts >= o.max_conflicts) return 2;
+            if (o.max_ticks > 0 && (J.conflicts & 63) == 0 && (int64_t)(wall_clock64() - t_start) > o.max_ticks) return 2;
+        } else {
+            if (restart_cnt >= restart_thr) {
+                restart_thr = (int32_t)((float)restart_thr * o.restart_factor);
+                restart_cnt = 0;
+                backtrack(J, 0, o.decision_mode == 0);
+                J.restarts++;
+                if (load_status(&a.inst_status[jd.inst]) != 2) return 3;
+                if (J.nlearnt * 2 >= J.learnt_cap || J.larena_used * 2 >= J.larena_cap) reduce_db(J, 0);
+                continue;
+            }
+            const int32_t v = pick_branch(J, o.decision_mode);
+            if (v < 0) return 0;
+            J.decisions++;
+            if (o.max_decisions > 0 && J.decisions > o.max_decisions) return 2;
+            if ((J.decisions & 1023) == 0) {
+                if (load_status(&a.inst_status[jd.inst]) != 2) return 3;
+                if (o.max_ticks > 0 && (int64_t)(wall_clock64() - t_start) > o.max_ticks) return 2;
+            }
+            J.dl++;
+            if (J.dl > J.levelmax) J.levelmax = J.dl;
+            J.tlim[J.dl] = J.tsize;
+            const int32_t ph = o.decision_mode == 1 ? 1 : (int32_t)J.phase[v];
+            assign(J, 2 * v + ph, -1);
+        }
+    }
+}
+
+}  // namespace
+
+template <bool kLds>
+__global__ __launch_bounds__(64) void k_solve(SolveArgs a)
+{
+    extern __shared__ __align__(16) char smem[];
+    const int lane = lane_id();
+    Ctx J;
+    carve_state(J, smem, a.max_n);
+    int32_t* cpart = kLds ? (int32_t*)(smem + Layout::state_bytes(a.max_n))
+                          : a.ws + (int64_t)blockIdx.x * a.ws_ints_per_wave;
+    carve_clauses(J, cpart, a);
+    for (;;) {
+        int32_t j = 0;
+        if (lane == 0) j = atomicAdd(a.queue_head, 1);
+        j = uni(__shfl(j, 0));
+        if (j >= a.n_jobs) break;
+        const JobDesc jd = a.jobs[j];
+        const int32_t st = run_job(J, a, jd);
+        __syncthreads();
+        const int32_t inst = jd.inst;
+        if (st == 0) {
+            int32_t won = 0;
+            if (lane == 0) won = atomicCAS(&a.inst_status[inst], 2, 0) == 2;
+            won = uni(__shfl(won, 0));
+            if (won) {
+                const InstMeta mi = a.meta[inst];
+                const int32_t W = words_per_row(mi.n);
+                for (int32_t w = lane; w < W; w += kWave) {
+                    uint32_t x = 0;
+                    for (int b = 0; b < 32; b++) {
+                        const int32_t l = 32 * w + b;
+                        if (l < 2 * mi.n && J.lv[l] == 1) x |= 1u << b;
+                    }
+                    a.model_rows[mi.word_base + w] = x;
+                }
+            }
+        } else if (st == 1) {
+            if (lane == 0) {
+                const int32_t r = atomicAdd(&a.inst_refuted[inst], 1) + 1;
+                if (r == a.inst_ncubes[inst]) atomicCAS(&a.inst_status[inst], 2, 1);
+            }
+        } else if (st == 2) {
+            if (lane == 0) atomicAdd(&a.inst_undef[inst], 1);
+        }
+        if (a.var_bumps) {
+            const int64_t vb = a.var_base[inst];
+            for (int32_t v = lane; v < J.n; v += kWave)
+                if (J.bumps[v]) atomicAdd(&a.var_bumps[vb + v], J.bumps[v]);
+        }
+        if (a.job_stats && lane == 0) {
+            int64_t* s = a.job_stats + (int64_t)jd.order * JS_COUNT;
+            s[JS_STATUS] = st;
+            s[JS_CONFLICTS] = J.conflicts;
+            s[JS_DECISIONS] = J.decisions;
+            s[JS_PROPS] = J.props;
+            s[JS_LEARNED] = J.learned;
+            s[JS_RESTARTS] = J.restarts;
+            s[JS_LEVELMAX] = J.levelmax;
+            s[7] = 0;
+        }
+        __syncthreads();
+    }
+}
+
+size_t solve_state_lds_bytes(int32_t max_n) { return Layout::state_bytes(max_n); }
+
+hipError_t launch_solve(const SolveArgs& a, bool lds_clauses, int waves, size_t lds_bytes, hipStream_t st)
+{
+    if (lds_clauses)
+        hipLaunchKernelGGL(k_solve<true>, dim3(waves), dim3(kWave), lds_bytes, st, a);
+    else
+        hipLaunchKernelGGL(k_solve<false>, dim3(waves), dim3(kWave), lds_bytes, st, a);
+    return hipGetLastError();
+}
+
+}  // namespace gpsat

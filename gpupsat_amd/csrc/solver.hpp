@@ -1,0 +1,65 @@
+// solver.hpp -- structures shared by the device CDCL solver and its host launcher.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.hpp"
+
+namespace gpsat {
+
+// Solver options (the strategy surfaces of the reference's compile-time Configs.cuh,
+// chosen at run time here).
+struct SolveOptsDev {
+    int32_t decision_mode;    // 0: VSIDS + phase saving; 1: reference scheme (last free var, positive)
+    int32_t restart_first;    // conflicts before the first restart (GEOMETRIC_CONFLICTS_BEFORE_RESTART)
+    float restart_factor;     // geometric growth (GEOMETRIC_RESTART_INCREASE_FACTOR)
+    int32_t learnt_cap;       // learned clauses held per job
+    int32_t larena_cap;       // int32 slots of learned-literal storage per job
+    int32_t watch_slack;      // learned-clause watch slots per literal
+    int64_t max_conflicts;    // per-job conflict budget (<= 0: none) -> UNDEF
+    int64_t max_decisions;    // per-job decision cap (<= 0: none) -> UNDEF (MAX_ITERATIONS analogue)
+    float var_decay;          // VSIDS activity decay per conflict
+    int32_t phase_init;       // initial phase when no hint is given: 0 false, 1 true
+    int64_t max_ticks;        // per-job wall-clock budget in device clock ticks (<= 0: none) -> UNDEF
+};
+
+struct JobDesc {
+    int32_t inst;
+    int32_t cube_off;
+    int32_t cube_len;
+    int32_t order;            // position of the job in the caller's job list
+};
+
+enum JobStat { JS_STATUS = 0, JS_CONFLICTS, JS_DECISIONS, JS_PROPS, JS_LEARNED, JS_RESTARTS, JS_LEVELMAX, JS_COUNT = 8 };
+
+struct SolveArgs {
+    const InstMeta* meta;
+    const int32_t* lits;
+    const int32_t* coff;
+    const JobDesc* jobs;
+    const int32_t* cube_lits;
+    int32_t n_jobs;
+    int32_t n_inst;
+    const int64_t* var_base;        // [n_inst] first var slot of each instance in per-var arrays
+    const float* act_init;          // [sum n] initial VSIDS scores (null: zero)
+    const int8_t* phase_init;       // [sum n] initial phases 0/1 (null: opts.phase_init)
+    int32_t* queue_head;            // device work-queue counter
+    int32_t* inst_status;           // [n_inst] 2 undecided, 0 SAT, 1 UNSAT
+    int32_t* inst_refuted;          // [n_inst] refuted cubes
+    int32_t* inst_undef;            // [n_inst] cubes stopped by a budget
+    const int32_t* inst_ncubes;     // [n_inst]
+    uint32_t* model_rows;           // [total_words] model of each SAT instance (literal bitmap)
+    int64_t* job_stats;             // [n_jobs * JS_COUNT]
+    int32_t* var_bumps;             // [sum n] conflict-analysis participation per var (null: off)
+    int32_t* ws;                    // global workspace
+    int64_t ws_ints_per_wave;
+    // per-wave clause-part layout (ints), identical in LDS and global mode
+    int32_t wo_cap, wl_cap, wpair_cap, lmeta_cap;
+    int32_t max_n;
+    SolveOptsDev o;
+};
+
+size_t solve_state_lds_bytes(int32_t max_n);
+hipError_t launch_solve(const SolveArgs& a, bool lds_clauses, int waves, size_t lds_bytes, hipStream_t st);
+
+}  // namespace gpsat

@@ -96,6 +96,55 @@ int gpsat_bcp_fixpoint(gpsat_batch *b, const uint32_t *rows, int32_t A, uint32_t
 int gpsat_eval_bcp_dev(gpsat_batch *b, const uint32_t *rows_dev, int32_t A, int32_t *counts_dev,
                        uint32_t *out_rows_dev, int32_t *conflict_dev, int32_t *props_dev);
 
+/* ---- a6..a13, a15: the solve path.
+ * Replaces the reference's solve kernels run_sequential / parallel_kernel
+ * (src/SATSolver/Parallelizer.cu:180-278, launched from src/SATSolver/main.cu:211,256-269):
+ * one wavefront per job, a persistent grid pulling jobs from a device queue.  A job is
+ * an instance plus a cube of literals assumed true (JobsQueue::next_job, JobsQueue.cu:10-32);
+ * an instance is SAT when one of its jobs is SAT and UNSAT only when every one of its
+ * jobs is refuted.  The strategy surfaces of the reference (Configs.cuh) are options. */
+typedef struct gpsat_solve_opts {
+    int32_t decision;        /* 0: VSIDS + phase saving; 1: reference rule, last free var positive
+                                (src/SATSolver/DecisionMaker.cu:45-55) */
+    int32_t restart_first;   /* conflicts before the first restart (reference 100, Configs.cuh:104) */
+    float restart_factor;    /* geometric growth (reference 1.3, Configs.cuh:105) */
+    int32_t learnt_cap;      /* learned clauses held per job (reference 20, Configs.cuh:94) */
+    int32_t larena_cap;      /* learned-literal slots per job (0: automatic) */
+    int32_t watch_slack;     /* learned-clause watch slots per literal (0: automatic) */
+    int64_t max_conflicts;   /* per-job conflict budget (<= 0: none), exceeded -> UNDEF */
+    int64_t max_decisions;   /* per-job decision cap, MAX_ITERATIONS analogue (Configs.cuh:23) */
+    float var_decay;         /* VSIDS decay per conflict */
+    int32_t phase_init;      /* initial phase without a hint: 0 false, 1 true */
+    double max_ms;           /* per-job wall-clock budget in ms (<= 0: none), exceeded -> UNDEF */
+    int32_t waves;           /* persistent wavefronts (0: automatic) */
+    int32_t clause_mem;      /* clause part of the job state: 0 automatic, 1 LDS, 2 HBM */
+    int32_t verify;          /* 1: re-check every SAT model with the clause-evaluation kernel */
+} gpsat_solve_opts;
+
+typedef struct gpsat_solve_info {
+    double kernel_ms;        /* solve kernel (HIP events on the batch stream) */
+    double verify_ms;        /* model re-verification kernel */
+    int32_t waves;
+    int32_t clause_mem;      /* 1 LDS, 2 HBM */
+    int64_t lds_bytes;       /* per wavefront */
+    int64_t workspace_bytes;
+    int32_t n_sat, n_unsat, n_undef, n_verified, n_errors;
+} gpsat_solve_info;
+
+int gpsat_solve_opts_default(gpsat_solve_opts *o);
+/* job_inst: instance of each job (NULL: one job per instance, no cubes); cube_off[n_jobs+1]
+ * and cube_lits: the literals of each job's cube.  act_init / phase_init: per-var
+ * initial VSIDS scores and phases, instances back to back (NULL: none).  Outputs:
+ * status[n_inst] (0 SAT, 1 UNSAT, 2 UNDEF); model_rows (gpsat_batch_row_words(b, 1)
+ * words: one literal bitmap per instance, valid where SAT); verified[n_inst] (1 where a
+ * SAT model was re-checked on the GPU); job_stats[n_jobs*8] (status, conflicts,
+ * decisions, propagations, learned, restarts, max level, 0); var_bumps (per var, how
+ * often conflict analysis met it).  Every output may be NULL except status. */
+int gpsat_solve(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t *job_inst, const int32_t *cube_off,
+                const int32_t *cube_lits, int32_t n_jobs, const float *act_init, const int8_t *phase_init,
+                int32_t *status, uint32_t *model_rows, int32_t *verified, int64_t *job_stats, int32_t *var_bumps,
+                gpsat_solve_info *info);
+
 #ifdef __cplusplus
 }
 #endif

@@ -1,0 +1,143 @@
+"""Verdict parity of the device CDCL solver (through gpsat_solve) with the oracle's
+solvers, GPU re-verification of every SAT model, and the cube/job-queue protocol."""
+import glob
+import os
+
+import numpy as np
+import pytest
+
+import gpupsat_amd as G
+from gpupsat_amd import workloads as W
+from gpupsat_amd.formula import Batch, read_dimacs
+
+from conftest import GOLDEN
+
+pytestmark = pytest.mark.gpu
+
+BUDGET_MS = 4000.0
+
+
+def opts(**kw):
+    kw.setdefault("max_ms", BUDGET_MS)
+    return G.SolveOpts.default(**kw)
+
+
+def truth(orc, f):
+    st, _, _ = orc.solve_cdcl(f)
+    return st
+
+
+def check_models(orc, gb, res, b):
+    models = gb.models(res["model_rows"])
+    for i in range(b.n_inst):
+        if res["status"][i] == G.SAT:
+            assert res["verified"][i] == 1
+            assert orc.verify_model(b.instance(i), models[i]), i
+
+
+def test_reference_fixtures(orc):
+    print("[solve] reference fixtures", flush=True)
+    paths = sorted(glob.glob(os.path.join(GOLDEN, "cnf", "*.cnf")))
+    fs = [read_dimacs(p) for p in paths]
+    exp = [{"SAT": 0, "UNSAT": 1}[open(p[:-4] + ".expected").read().strip()] for p in paths]
+    b = Batch.from_formulas(fs)
+    for mem in (1, 2):
+        with G.GpuBatch(b) as gb:
+            res = gb.solve(opts(clause_mem=mem))
+            assert res["status"].tolist() == exp
+            check_models(orc, gb, res, b)
+
+
+@pytest.mark.parametrize("mem", [1, 2])
+def test_config1_verdicts(orc, mem):
+    print(f"[solve] config-1 100 x n=50 m=213 clause_mem={mem}", flush=True)
+    b = W.random_3sat_batch(100, 50, 213, seed=0)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(clause_mem=mem))
+        check_models(orc, gb, res, b)
+    exp = [truth(orc, b.instance(i)) for i in range(b.n_inst)]
+    assert res["status"].tolist() == exp
+    assert res["info"]["clause_mem"] == mem
+
+
+def test_reference_decision_rule(orc):
+    print("[solve] reference decision rule", flush=True)
+    b = W.random_3sat_batch(40, 40, 170, seed=3)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(decision=1, learnt_cap=20))
+        check_models(orc, gb, res, b)
+    assert res["status"].tolist() == [truth(orc, b.instance(i)) for i in range(b.n_inst)]
+
+
+def test_tiny_learned_db_and_watch_slack(orc):
+    """learnt_cap 4 and one learned watch slot per literal force the DB-full restart
+    and the watch-overflow detach paths."""
+    print("[solve] tiny learned DB", flush=True)
+    b = W.random_3sat_batch(48, 60, 256, seed=5)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(learnt_cap=4, larena_cap=200, watch_slack=1))
+        check_models(orc, gb, res, b)
+    assert res["status"].tolist() == [truth(orc, b.instance(i)) for i in range(b.n_inst)]
+
+
+def test_pigeonhole_and_coloring(orc):
+    print("[solve] pigeonhole + colouring", flush=True)
+    fs = [W.pigeonhole_formula(p + 1, p) for p in (3, 4, 5, 6)]
+    fs += [W.pigeonhole_formula(5, 5)]
+    fs += [W.coloring_batch(4, n_nodes=60, seed=2).instance(i) for i in range(4)]
+    b = Batch.from_formulas(fs)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts())
+        check_models(orc, gb, res, b)
+    st = res["status"].tolist()
+    assert st[:4] == [G.UNSAT] * 4          # PHP(p+1, p) is UNSAT (theorem)
+    assert st[4] == G.SAT
+    assert st[5:] == [truth(orc, f) for f in fs[5:]]
+
+
+def test_cubes_cover_the_search_space(orc):
+    """Every instance split into 2^d cubes: SAT iff some cube is SAT, UNSAT only when
+    every cube is refuted (the reference's parallel path can stop early,
+    src/SATSolver/Parallelizer.cu:60-74)."""
+    print("[solve] cube jobs", flush=True)
+    b = W.random_3sat_batch(30, 50, 213, seed=11)
+    ji, co, cl = [], [0], []
+    for i in range(b.n_inst):
+        f = b.instance(i)
+        vs = orc.choose_cube_vars(f, 0, 1, 1, 0)          # d = 4 -> 16 cubes
+        for cube in orc.enumerate_cubes(vs):
+            ji.append(i)
+            cl.extend(cube.tolist())
+            co.append(len(cl))
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(), jobs=(ji, co, cl))
+        check_models(orc, gb, res, b)
+    assert res["status"].tolist() == [truth(orc, b.instance(i)) for i in range(b.n_inst)]
+    js = res["job_stats"]
+    for i in range(b.n_inst):
+        mine = js[np.array(ji) == i][:, 0]
+        if res["status"][i] == G.UNSAT:
+            assert (mine == G.UNSAT).all()
+        else:
+            assert (mine == G.SAT).any()
+
+
+def test_budget_gives_undef(orc):
+    print("[solve] budget", flush=True)
+    b = Batch.from_formulas([W.pigeonhole_formula(9, 8)])
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(max_conflicts=3))
+    assert res["status"].tolist() == [G.UNDEF]
+    assert res["job_stats"][0, 0] == G.UNDEF
+
+
+def test_mixed_sizes_with_budget(orc):
+    print("[solve] mixed k-SAT n in [100,400]", flush=True)
+    b, ks = W.mixed_ksat_batch(24, seed=1)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(max_conflicts=20000))
+        check_models(orc, gb, res, b)
+    for i in range(b.n_inst):
+        if res["status"][i] == G.UNSAT:
+            st, _, _ = orc.solve_cdcl(b.instance(i), max_conflicts=2_000_000)
+            assert st == G.UNSAT
