@@ -18,6 +18,8 @@ import os
 import sys
 import time
 
+import numpy as np
+
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
@@ -56,6 +58,81 @@ def cpu_baseline_eval_bcp(batch, rows, A, seconds: float):
     return done / dt, done, dt
 
 
+def _ref_solve_chunk(sub):
+    """Worker: the oracle's C restatement of the reference's sequential path
+    (preprocessing, then decision = last free var, decision-scheme learning, 20-clause
+    learned DB, restarts 100/x1.3) over a slice of instances, one at a time."""
+    import oracle
+    return oracle.solve_ref_batch(sub)
+
+
+def cpu_baseline_solve(batch, seconds: float, cores: int):
+    """All-core CPU baseline of the solve leg: P worker processes, each solving its
+    share sequentially (-b 1 -t 1 semantics).  Returns instances/s over the sample."""
+    import multiprocessing as mp
+    import oracle
+    # size the sample from a short single-core probe so the leg takes about `seconds`
+    probe = batch.subset(range(min(64, batch.n_inst)))
+    t0 = time.perf_counter()
+    oracle.solve_ref_batch(probe)
+    per = (time.perf_counter() - t0) / probe.n_inst
+    n_sample = int(min(batch.n_inst, max(cores, seconds * cores / max(per, 1e-7))))
+    chunks = [batch.subset(range(s, min(n_sample, s + 64))) for s in range(0, n_sample, 64)]
+    ctx = mp.get_context("fork")
+    with ctx.Pool(cores) as pool:
+        t0 = time.perf_counter()
+        res = pool.map(_ref_solve_chunk, chunks, chunksize=1)
+        dt = time.perf_counter() - t0
+    return n_sample / dt, n_sample, dt, np.concatenate(res)
+
+
+def solve_leg(G, W, b, cpu):
+    """Verified random 3-SAT instances/s: N uniform random 3-SAT instances (n = 50,
+    m = 213), one wavefront per instance; every SAT model re-verified on the GPU."""
+    import oracle
+    N = b.n_inst
+    gb = G.GpuBatch(b)
+    o = G.SolveOpts.default(max_ms=30000)
+    warm = G.GpuBatch(b.subset(range(min(256, N))))
+    warm.solve(o)
+    warm.close()
+    t0 = time.perf_counter()
+    res = gb.solve(o, want_models=False)
+    wall = time.perf_counter() - t0
+    inf = res["info"]
+    st = res["status"]
+    decided = int((st != 2).sum())
+    verified_sat = int(res["verified"].sum())
+    ms = inf["kernel_ms"] + inf["verify_ms"]
+    # verdict agreement on a sample, against the oracle's textbook CDCL
+    idx = list(range(0, N, max(1, N // 200)))[:200]
+    agree = sum(int(oracle.solve_cdcl(b.instance(i))[0] == st[i]) for i in idx)
+    js = res["job_stats"]
+    leg = {
+        "workload": f"{N} uniform random 3-SAT (n=50, m=213), one wavefront per instance",
+        "verified_instances_per_s": (verified_sat + int((st == 1).sum())) / (ms * 1e-3),
+        "kernel_ms": inf["kernel_ms"], "verify_ms": inf["verify_ms"], "wall_s": wall,
+        "sat": int((st == 0).sum()), "unsat": int((st == 1).sum()), "undef": int((st == 2).sum()),
+        "sat_models_verified_on_gpu": verified_sat, "decided": decided,
+        "verdict_agreement_sample": f"{agree}/{len(idx)}",
+        "mean_conflicts": float(js[:, 1].mean()), "mean_props": float(js[:, 3].mean()),
+        "props_per_s": float(js[:, 3].sum()) / (inf["kernel_ms"] * 1e-3),
+        "waves": inf["waves"], "clause_mem": "LDS" if inf["clause_mem"] == 1 else "HBM",
+    }
+    if cpu is not None:
+        rate, ns, dt, cres, cores = cpu
+        agree_cpu = sum(int(cres[i] == st[i]) for i in range(ns) if cres[i] != 2)
+        leg["cpu_baseline"] = {
+            "value": rate, "unit": "instances/s", "cores": cores, "kind": "port",
+            "sample": f"first {ns} of the same instances, {dt:.1f} s wall on {cores} host cores "
+                      f"(reference scheme restated in oracle/, one process per core); "
+                      f"{agree_cpu}/{sum(1 for r in cres if r != 2)} decided verdicts agree with the GPU",
+        }
+        leg["speedup_vs_cpu"] = leg["verified_instances_per_s"] / rate
+    gb.close()
+    return leg
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -64,15 +141,25 @@ def main():
     ap.add_argument("--instances", type=int, default=10_000)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--solve-instances", type=int, default=20_000)
+    ap.add_argument("--no-solve", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the instance batch would shard across GPUs with no "
               "exchange (see DESIGN.md); multi-GPU runs are not part of this build.", file=sys.stderr)
         sys.exit(2)
 
-    import numpy as np
     import gpupsat_amd as G
     from gpupsat_amd import workloads as W
+
+    # The multi-process CPU baseline of the solve leg runs first, before this process
+    # touches the GPU, so the forked workers inherit no device state.
+    solve_batch = solve_cpu = None
+    if not args.no_solve:
+        solve_batch = W.random_3sat_batch(args.solve_instances, 50, 213, seed=100)
+        if not args.no_cpu:
+            cores = max(1, min(16, len(os.sched_getaffinity(0))))
+            solve_cpu = cpu_baseline_solve(solve_batch, args.cpu_seconds, cores) + (cores,)
 
     N, n, m, A, p = args.instances, 200, 852, 64, 0.3
     b = W.random_3sat_batch(N, n, m, seed=0)
@@ -177,6 +264,10 @@ def main():
                "sample": f"oracle clause-eval + UP-closure of the first {done} instances x 64 rows "
                          f"(same workload), {dt:.1f} s on one host core"}
 
+    legs = {}
+    if solve_batch is not None:
+        legs["solve"] = solve_leg(G, W, solve_batch, solve_cpu)
+
     out = {
         "metric": METRIC,
         "value": N / (ms_per_step * 1e-3),
@@ -209,6 +300,7 @@ def main():
             "p0.05_props_per_s": float(props_lo.sum()) / (ms_bcp_lo * 1e-3),
         },
         "parity_sample_ok": parity,
+        "legs": legs,
     }
     print(json.dumps(out))
 

@@ -117,6 +117,91 @@ class Event:
             pass
 
 
+# ---- host-side boundary functions (no GPU needed)
+def read_dimacs(path: str):
+    """DIMACS through the library's reader: (Formula, header_vars, header_clauses)."""
+    f = ctypes.c_void_p()
+    _check(lib().gpsat_dimacs_read(path.encode(), ctypes.byref(f)))
+    return _take_formula(f)
+
+
+def parse_dimacs(text: str):
+    f = ctypes.c_void_p()
+    b = text.encode()
+    _check(lib().gpsat_dimacs_parse(b, ctypes.c_int64(len(b)), ctypes.byref(f)))
+    return _take_formula(f)
+
+
+def _take_formula(f):
+    n, m, hv, hc = (ctypes.c_int32() for _ in range(4))
+    L = ctypes.c_int64()
+    lib().gpsat_formula_info(f, ctypes.byref(n), ctypes.byref(m), ctypes.byref(L), ctypes.byref(hv), ctypes.byref(hc))
+    lits = np.zeros(max(L.value, 1), dtype=np.int32)
+    off = np.zeros(m.value + 1, dtype=np.int64)
+    lib().gpsat_formula_arrays(f, _p(lits), _p(off))
+    lib().gpsat_formula_free(f)
+    return Formula(n.value, lits[: L.value], off), hv.value, hc.value
+
+
+def preprocess(f: Formula):
+    """The reference's preprocessing: (status, Formula or None, solved literals)."""
+    L = max(len(f.lits), 1)
+    out_lits = np.zeros(L, dtype=np.int32)
+    out_off = np.zeros(f.n_clauses + 1, dtype=np.int64)
+    solved = np.zeros(f.n_clauses + 1, dtype=np.int32)
+    om, ns, st = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+    lits = np.ascontiguousarray(f.lits, dtype=np.int32)
+    off = np.ascontiguousarray(f.clause_off, dtype=np.int64)
+    _check(lib().gpsat_preprocess(_p(lits), _p(off), ctypes.c_int32(f.n_clauses), ctypes.c_int32(f.n_vars),
+                                  _p(out_lits), _p(out_off), ctypes.byref(om), _p(solved), ctypes.byref(ns),
+                                  ctypes.byref(st)))
+    g = None
+    if st.value == UNDEF:
+        g = Formula(f.n_vars, out_lits[: out_off[om.value]].copy(), out_off[: om.value + 1].copy())
+    return st.value, g, solved[: ns.value].copy()
+
+
+def cube_vars(f: Formula, threads: int = 32, blocks: int = 32, strategy: int = 0, dead=()):
+    out = np.zeros(16, dtype=np.int32)
+    d = ctypes.c_int32()
+    dv = np.ascontiguousarray(list(dead) or [0], dtype=np.int32)
+    lits = np.ascontiguousarray(f.lits, dtype=np.int32)
+    off = np.ascontiguousarray(f.clause_off, dtype=np.int64)
+    _check(lib().gpsat_cube_vars(_p(lits), _p(off), ctypes.c_int32(f.n_clauses), ctypes.c_int32(f.n_vars), _p(dv),
+                                 ctypes.c_int32(len(dead)), ctypes.c_int64(threads), ctypes.c_int64(blocks),
+                                 ctypes.c_int32(strategy), _p(out), ctypes.byref(d)))
+    return out[: d.value].copy()
+
+
+def cube_vars_scored(scores, depth: int, dead=()):
+    s = np.ascontiguousarray(scores, dtype=np.float32)
+    out = np.zeros(max(depth, 1), dtype=np.int32)
+    d = ctypes.c_int32()
+    dv = np.ascontiguousarray(list(dead) or [0], dtype=np.int32)
+    _check(lib().gpsat_cube_vars_scored(_p(s), ctypes.c_int32(len(s)), _p(dv), ctypes.c_int32(len(dead)),
+                                        ctypes.c_int32(depth), _p(out), ctypes.byref(d)))
+    return out[: d.value].copy()
+
+
+def cubes(vars_) -> np.ndarray:
+    v = np.ascontiguousarray(vars_, dtype=np.int32)
+    d = len(v)
+    out = np.zeros(((1 << d), max(d, 1)), dtype=np.int32)
+    _check(lib().gpsat_cubes(_p(v) if d else None, ctypes.c_int32(d), _p(out)))
+    return out[:, :d]
+
+
+def cube_jobs(batch: Batch, vars_per_inst) -> tuple:
+    """Job arrays (job_inst, cube_off, cube_lits) splitting instance i on vars_per_inst[i]."""
+    ji, co, cl = [], [0], []
+    for i, vs in enumerate(vars_per_inst):
+        for cube in cubes(vs):
+            ji.append(i)
+            cl.extend(cube.tolist())
+            co.append(len(cl))
+    return (np.array(ji, dtype=np.int32), np.array(co, dtype=np.int32), np.array(cl or [0], dtype=np.int32))
+
+
 class SolveOpts(ctypes.Structure):
     """Mirror of gpsat_solve_opts (include/gpsat.h)."""
     _fields_ = [

@@ -1,6 +1,7 @@
 // capi.cpp -- the extern "C" boundary of libgpsat.so: batches, clause evaluation,
 // BCP fixpoints and the device-memory / timing helpers.
 #include <hip/hip_runtime.h>
+#include <stdlib.h>
 #include <string.h>
 
 #include <algorithm>
@@ -12,6 +13,16 @@
 namespace gpsat {
 static thread_local std::string g_last_error;
 void set_error(const std::string& msg) { g_last_error = msg; }
+
+Tuning& tuning()
+{
+    static Tuning t = [] {
+        Tuning x;
+        if (const char* e = getenv("GPSAT_TUNE_EVAL_STAGE")) x.eval_stage_arena = atoi(e);
+        return x;
+    }();
+    return t;
+}
 }  // namespace gpsat
 
 using namespace gpsat;

@@ -32,30 +32,41 @@ __device__ __forceinline__ uint32_t swz_xor(uint32_t x)
     return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (S << 10));
 }
 
-// One stage of the 32x32 bit-matrix transpose held one row per lane: swap the
-// off-diagonal SxS blocks of every 2Sx2S block.
-template <int S>
-__device__ __forceinline__ uint32_t tstage(uint32_t x, bool upper)
-{
-    constexpr uint32_t H = S == 16 ? 0xFFFF0000u : S == 8 ? 0xFF00FF00u : S == 4 ? 0xF0F0F0F0u
-                         : S == 2 ? 0xCCCCCCCCu : 0xAAAAAAAAu;
-    uint32_t y = swz_xor<S>(x);
-    uint32_t t = upper ? (y >> S) : (y << S);
-    uint32_t keep = upper ? H : ~H;
-    return (x & keep) | (t & ~keep);
-}
-
-// Transpose the 32x32 bit matrix whose row r is word x of lane (r mod 32) within
-// each 32-lane half: afterwards lane r holds column r (bit c = old row c, bit r).
-__device__ __forceinline__ uint32_t transpose32(uint32_t x, int lane)
-{
-    x = tstage<16>(x, lane & 16);
-    x = tstage<8>(x, lane & 8);
-    x = tstage<4>(x, lane & 4);
-    x = tstage<2>(x, lane & 2);
-    x = tstage<1>(x, lane & 1);
-    return x;
-}
+// 32x32 bit-matrix transpose held one row per lane in each 32-lane half: stage S swaps
+// the off-diagonal SxS blocks of every 2Sx2S block.  Per stage a lane takes its partner's
+// row (lane ^ S), rotates it by S one way or the other (a per-lane rotate amount) and
+// keeps its own bits outside the half it receives.
+struct Transposer {
+    uint32_t rot[5];
+    uint32_t keep[5];
+    __device__ explicit Transposer(int lane)
+    {
+        const uint32_t H[5] = {0xFFFF0000u, 0xFF00FF00u, 0xF0F0F0F0u, 0xCCCCCCCCu, 0xAAAAAAAAu};
+#pragma unroll
+        for (int k = 0; k < 5; k++) {
+            const uint32_t S = 16u >> k;
+            const bool upper = (lane & S) != 0;
+            rot[k] = upper ? S : 32u - S;       // rotate right by S, or left by S
+            keep[k] = upper ? H[k] : ~H[k];
+        }
+    }
+    template <int K>
+    __device__ __forceinline__ uint32_t stage(uint32_t x) const
+    {
+        const uint32_t y = swz_xor<(16 >> K)>(x);
+        const uint32_t t = __builtin_amdgcn_alignbit(y, y, rot[K]);
+        return (x & keep[K]) | (t & ~keep[K]);
+    }
+    // afterwards lane r holds column r (bit c = old row c, bit r)
+    __device__ __forceinline__ uint32_t operator()(uint32_t x) const
+    {
+        x = stage<0>(x);
+        x = stage<1>(x);
+        x = stage<2>(x);
+        x = stage<3>(x);
+        return stage<4>(x);
+    }
+};
 
 __device__ __forceinline__ uint32_t shfl_xor32(uint32_t x)
 {

@@ -31,6 +31,12 @@ struct DeviceBatch {
 
 void set_error(const std::string& msg);
 
+// Launch-shape choices, set by the autotuner (gpsat_tune_*) or the GPSAT_TUNE_* environment.
+struct Tuning {
+    int eval_stage_arena = 0;     // copy each instance's clause arena to LDS before evaluating
+};
+Tuning& tuning();
+
 size_t eval_bcp_lds_bytes(int max_n);
 hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, int32_t* counts, uint32_t* out_rows,
                            int32_t* conflict, int32_t* props, int32_t* passes, bool do_eval, bool do_bcp,

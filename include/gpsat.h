@@ -96,6 +96,41 @@ int gpsat_bcp_fixpoint(gpsat_batch *b, const uint32_t *rows, int32_t A, uint32_t
 int gpsat_eval_bcp_dev(gpsat_batch *b, const uint32_t *rows_dev, int32_t A, int32_t *counts_dev,
                        uint32_t *out_rows_dev, int32_t *conflict_dev, int32_t *props_dev);
 
+/* ---- host-side input: DIMACS CNF, the reader of src/FileManager/CnfReader.cpp:30-135
+ * (CnfManager::read_cnf).  n_vars is the highest var seen (CnfReader.cpp:117-123);
+ * the header's counts are kept for the reference's mismatch warnings. */
+typedef struct gpsat_formula gpsat_formula;
+int gpsat_dimacs_read(const char *path, gpsat_formula **out);
+int gpsat_dimacs_parse(const char *text, int64_t len, gpsat_formula **out);
+int gpsat_formula_info(const gpsat_formula *f, int32_t *n_vars, int32_t *n_clauses, int64_t *n_lits,
+                       int32_t *header_vars, int32_t *header_clauses);
+/* lits[n_lits] and clause_off[n_clauses + 1] */
+int gpsat_formula_arrays(const gpsat_formula *f, int32_t *lits, int64_t *clause_off);
+int gpsat_formula_free(gpsat_formula *f);
+
+/* ---- a3: preprocessing, FormulaData::copy_host_clauses_to_dev
+ * (src/FileManager/FormulaData.cu:82-105): RepeatedLiteralsRemover then
+ * UnaryClausesRemover.  Output buffers hold at least the input sizes; solved holds
+ * up to m literals.  status: 0 SAT, 1 UNSAT (decided by preprocessing), 2 formula remains. */
+int gpsat_preprocess(const int32_t *lits, const int64_t *clause_off, int32_t m, int32_t n_vars, int32_t *out_lits,
+                     int64_t *out_clause_off, int32_t *out_m, int32_t *solved, int32_t *n_solved, int32_t *status);
+
+/* ---- a14: cube generation, the JobChooser surface (src/JobsManager/JobChooser.cuh:21-31).
+ * strategy 0: MaxClauseJobChooser distributing jobs per thread, 1: MaxClauseJobChooser
+ * uniform (src/JobsManager/JobChooser.cu:52-133, VariableChooser.cu:23-40); 2:
+ * SimpleJobChooser (src/JobsManager/SimpleJobChooser.cu:22-49).  out_vars holds up to
+ * 15 vars; *depth gets their number. */
+int gpsat_cube_vars(const int32_t *lits, const int64_t *clause_off, int32_t m, int32_t n_vars, const int32_t *dead_vars,
+                    int32_t n_dead, int64_t threads, int64_t blocks, int32_t strategy, int32_t *out_vars,
+                    int32_t *depth);
+/* The depth highest-scoring live vars (for example the learned network's scores);
+ * no reference counterpart. */
+int gpsat_cube_vars_scored(const float *scores, int32_t n_vars, const int32_t *dead_vars, int32_t n_dead,
+                           int32_t depth, int32_t *out_vars, int32_t *out_depth);
+/* All 2^depth cubes over vars, depth first with the positive literal first
+ * (MaxClauseJobChooser::addJobs, JobChooser.cu:68-90): out_lits[2^depth * depth]. */
+int gpsat_cubes(const int32_t *vars, int32_t depth, int32_t *out_lits);
+
 /* ---- a6..a13, a15: the solve path.
  * Replaces the reference's solve kernels run_sequential / parallel_kernel
  * (src/SATSolver/Parallelizer.cu:180-278, launched from src/SATSolver/main.cu:211,256-269):

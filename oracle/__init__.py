@@ -113,6 +113,14 @@ def solve_ref(f, dead=None, max_iterations: int = 0):
     return st, model[: f.n_vars], stats
 
 
+def solve_ref_batch(batch, max_iterations: int = 0) -> np.ndarray:
+    status = np.zeros(batch.n_inst, dtype=np.int32)
+    lib().orc_solve_ref_batch(ctypes.c_int32(batch.n_inst), _p(_i32(batch.lits)), _p(_i64(batch.clause_off)),
+                              _p(_i64(batch.inst_clause_off)), _p(_i32(batch.n_vars)),
+                              ctypes.c_int32(max_iterations), _p(status))
+    return status
+
+
 def solve_dpll(f, max_nodes: int = 0):
     model = np.zeros(max(f.n_vars, 1), dtype=np.int8)
     st = lib().orc_solve_dpll(_p(_i32(f.lits)), _p(_i64(f.clause_off)), ctypes.c_int32(f.n_clauses),

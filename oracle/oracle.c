@@ -714,3 +714,34 @@ void orc_restart_schedule(int32_t first, float factor, int32_t k, int32_t *out)
     int32_t t = first;
     for (int32_t i = 0; i < k; i++) { out[i] = t; t = (int32_t)(t * factor); }
 }
+
+/* ------------------------------------------------------------------------------------------
+ * The reference's sequential path for a batch, one instance at a time, as its main does
+ * (src/SATSolver/main.cu:125-211): preprocessing (FormulaData.cu:82-105), an answer from
+ * preprocessing if it gives one (main.cu:154-163), else the sequential solver with the
+ * solved literals' vars as dead vars (main.cu:165-174).  status[i] gets the verdict. */
+void orc_solve_ref_batch(int32_t n_inst, const int32_t *lits, const int64_t *clause_off,
+                         const int64_t *inst_clause_off, const int32_t *n_vars, int32_t max_iterations,
+                         int32_t *status)
+{
+    for (int32_t i = 0; i < n_inst; i++) {
+        int64_t c0 = inst_clause_off[i], c1 = inst_clause_off[i + 1];
+        int32_t m = (int32_t)(c1 - c0), n = n_vars[i];
+        int64_t L = clause_off[c1] - clause_off[c0];
+        int32_t *pl = (int32_t *)malloc(sizeof(int32_t) * (L ? L : 1));
+        int64_t *po = (int64_t *)malloc(sizeof(int64_t) * (m + 1));
+        int64_t *io = (int64_t *)malloc(sizeof(int64_t) * (m + 1));
+        int32_t *solved = (int32_t *)malloc(sizeof(int32_t) * (m + 1));
+        for (int32_t c = 0; c <= m; c++) io[c] = clause_off[c0 + c] - clause_off[c0];
+        int32_t pm = 0, ns = 0;
+        int st = orc_preprocess(lits + clause_off[c0], io, m, n, pl, po, &pm, solved, &ns);
+        if (st == ORC_UNDEF) {
+            uint8_t *dead = (uint8_t *)calloc(n ? n : 1, 1);
+            for (int32_t k = 0; k < ns; k++) dead[solved[k] >> 1] = 1;
+            st = orc_solve_ref(pl, po, pm, n, dead, max_iterations, NULL, NULL);
+            free(dead);
+        }
+        status[i] = st;
+        free(pl); free(po); free(io); free(solved);
+    }
+}

@@ -74,6 +74,12 @@ int orc_preprocess(const int32_t *lits, const int64_t *clause_off, int32_t m, in
 int orc_solve_ref(const int32_t *lits, const int64_t *clause_off, int32_t m, int32_t n,
                   const uint8_t *dead, int32_t max_iterations, int8_t *model, int64_t *stats);
 
+/* The reference's sequential path (preprocessing, then the solver with solved vars as
+ * dead vars) for every instance of a batch; status[i] gets the verdict. */
+void orc_solve_ref_batch(int32_t n_inst, const int32_t *lits, const int64_t *clause_off,
+                         const int64_t *inst_clause_off, const int32_t *n_vars, int32_t max_iterations,
+                         int32_t *status);
+
 /* Independent checkers for verdicts (no reference counterpart). */
 int orc_solve_dpll(const int32_t *lits, const int64_t *clause_off, int32_t m, int32_t n,
                    int8_t *model, int64_t max_nodes);
