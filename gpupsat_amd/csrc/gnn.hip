@@ -1,0 +1,772 @@
+// gnn.hip -- the learned branching network: message passing over the literal-clause
+// incidence graph, trained and run on the GPU (no reference counterpart).
+//
+// Nodes: 2n literal nodes and m clause nodes per instance, instances back to back;
+// an edge per clause occurrence.  Hidden size 64, R rounds, weights shared by rounds:
+//   H_l^0 = occ(l)/8 u_l + v_l                       H_c^0 = len(c)/4 u_c + v_c
+//   M_c = 1/4 sum_{l in c} H_l     H_c' = relu([H_c | M_c] W_c^T + b_c)
+//   M_l = 1/8 sum_{c ni l} H_c'    H_l' = relu([H_l | M_l | H_{not l}] W_l^T + b_l)
+//   Z_v = [H_{2v+1} | H_{2v}]      R_v = relu(Z_v W_o^T + b_o)   Y_v = R_v W_y^T + b_y
+// Y_v = (branching score, phase logit).  The three 64-wide products run on MFMA
+// (v_mfma_f32_16x16x16_bf16): BF16 weights and BF16 inputs, FP32 accumulation.
+// Aggregations, biases, the 2-output head, the loss and Adam are FP32.
+#include <hip/hip_runtime.h>
+#include <string.h>
+
+#include <algorithm>
+#include <string>
+#include <vector>
+
+#include "../../include/gpsat.h"
+#include "common.hpp"
+#include "gpsat_internal.hpp"
+
+namespace gpsat {
+namespace gnn {
+
+constexpr int HID = 64;
+typedef uint16_t bf16_t;
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+// parameter layout (FP32 master copy), in this order
+constexpr int O_UL = 0, O_VL = 64, O_UC = 128, O_VC = 192;
+constexpr int O_WC = 256, O_BC = O_WC + 64 * 128, O_WL = O_BC + 64, O_BL = O_WL + 64 * 192;
+constexpr int O_WO = O_BL + 64, O_BO = O_WO + 64 * 128, O_WY = O_BO + 64, O_BY = O_WY + 2 * 64;
+constexpr int N_PARAMS = O_BY + 2;
+
+__device__ __forceinline__ bf16_t to_bf16(float f)
+{
+    uint32_t u = __float_as_uint(f);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (bf16_t)(u >> 16);
+}
+__device__ __forceinline__ float from_bf16(bf16_t h) { return __uint_as_float((uint32_t)h << 16); }
+
+// ---------------------------------------------------------------------------------------
+// MFMA tiles.  v_mfma_f32_16x16x16_bf16 on a wave of 64: lane l supplies
+// A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16] (j = 0..3) and receives D[4(l/16)+j][l%16].
+
+// out[N x 64] = relu(X[N x K] W[64 x K]^T + b); one wave per 16 rows, W staged in LDS.
+template <int K>
+__global__ __launch_bounds__(256) void k_dense_fwd(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wq,
+                                                   const float* __restrict__ bias, float* __restrict__ out, int N)
+{
+    __shared__ __align__(16) bf16_t sW[64 * K];
+    for (int i = threadIdx.x; i < 64 * K / 4; i += blockDim.x)
+        reinterpret_cast<uint2*>(sW)[i] = reinterpret_cast<const uint2*>(Wq)[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = (blockIdx.x * 4 + wave) * 16;
+    if (r0 >= N) return;
+    const int ar = r0 + (lane & 15), kq = 4 * (lane >> 4);
+    f32x4 acc[4];
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int k0 = 0; k0 < K; k0 += 16) {
+        bf16x4 a = {0, 0, 0, 0};
+        if (ar < N) a = *reinterpret_cast<const bf16x4*>(X + (size_t)ar * K + k0 + kq);
+#pragma unroll
+        for (int ct = 0; ct < 4; ct++) {
+            const bf16x4 b = *reinterpret_cast<const bf16x4*>(sW + (ct * 16 + (lane & 15)) * K + k0 + kq);
+            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < 4; ct++) {
+        const int col = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int row = r0 + kq + j;
+            if (row < N) out[(size_t)row * 64 + col] = fmaxf(acc[ct][j] + bias[col], 0.0f);
+        }
+    }
+}
+
+// dX[N x K] = dP[N x 64] W[64 x K]
+template <int K>
+__global__ __launch_bounds__(256) void k_dense_bwd_dx(const bf16_t* __restrict__ dP, const bf16_t* __restrict__ Wq,
+                                                      float* __restrict__ dX, int N)
+{
+    __shared__ __align__(16) bf16_t sW[64 * K];
+    for (int i = threadIdx.x; i < 64 * K / 4; i += blockDim.x)
+        reinterpret_cast<uint2*>(sW)[i] = reinterpret_cast<const uint2*>(Wq)[i];
+    __syncthreads();
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int r0 = (blockIdx.x * 4 + wave) * 16;
+    if (r0 >= N) return;
+    const int ar = r0 + (lane & 15), kq = 4 * (lane >> 4);
+    constexpr int CT = K / 16;
+    f32x4 acc[CT];
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int h0 = 0; h0 < 64; h0 += 16) {
+        bf16x4 a = {0, 0, 0, 0};
+        if (ar < N) a = *reinterpret_cast<const bf16x4*>(dP + (size_t)ar * 64 + h0 + kq);
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+            const int col = ct * 16 + (lane & 15);
+            bf16x4 b;
+            b[0] = (short)sW[(h0 + kq + 0) * K + col];
+            b[1] = (short)sW[(h0 + kq + 1) * K + col];
+            b[2] = (short)sW[(h0 + kq + 2) * K + col];
+            b[3] = (short)sW[(h0 + kq + 3) * K + col];
+            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct], 0, 0, 0);
+        }
+    }
+#pragma unroll
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = ct * 16 + (lane & 15);
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const int row = r0 + kq + j;
+            if (row < N) dX[(size_t)row * K + col] = acc[ct][j];
+        }
+    }
+}
+
+// gW[64 x K] += dP[N x 64]^T X[N x K]; one wave per block, 64-node chunks staged in LDS.
+template <int K>
+__global__ __launch_bounds__(64) void k_dense_bwd_dw(const bf16_t* __restrict__ dP, const bf16_t* __restrict__ X,
+                                                     float* __restrict__ gW, int N)
+{
+    __shared__ __align__(16) bf16_t sP[64 * 64];
+    __shared__ __align__(16) bf16_t sX[64 * K];
+    constexpr int CT = K / 16;
+    const int lane = threadIdx.x;
+    const int kq = 4 * (lane >> 4);
+    f32x4 acc[4][CT];
+#pragma unroll
+    for (int ht = 0; ht < 4; ht++)
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) acc[ht][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int n0 = blockIdx.x * 64; n0 < N; n0 += gridDim.x * 64) {
+        const int rows = min(64, N - n0);
+        __syncthreads();
+        for (int i = lane; i < 64 * 64 / 4; i += 64) {
+            const int r = (i * 4) / 64;
+            uint2 v = {0u, 0u};
+            if (r < rows) v = reinterpret_cast<const uint2*>(dP + (size_t)n0 * 64)[i];
+            reinterpret_cast<uint2*>(sP)[i] = v;
+        }
+        for (int i = lane; i < 64 * K / 4; i += 64) {
+            const int r = (i * 4) / K;
+            uint2 v = {0u, 0u};
+            if (r < rows) v = reinterpret_cast<const uint2*>(X + (size_t)n0 * K)[i];
+            reinterpret_cast<uint2*>(sX)[i] = v;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int s = 0; s < 64; s += 16) {
+            // A[h][n] = dP[n][h] (16 h x 16 n), B[n][col] = X[n][col]
+            bf16x4 a[4];
+#pragma unroll
+            for (int ht = 0; ht < 4; ht++) {
+                const int h = ht * 16 + (lane & 15);
+#pragma unroll
+                for (int j = 0; j < 4; j++) a[ht][j] = (short)sP[(s + kq + j) * 64 + h];
+            }
+#pragma unroll
+            for (int ct = 0; ct < CT; ct++) {
+                const int col = ct * 16 + (lane & 15);
+                bf16x4 b;
+#pragma unroll
+                for (int j = 0; j < 4; j++) b[j] = (short)sX[(s + kq + j) * K + col];
+#pragma unroll
+                for (int ht = 0; ht < 4; ht++)
+                    acc[ht][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ht], b, acc[ht][ct], 0, 0, 0);
+            }
+        }
+    }
+#pragma unroll
+    for (int ht = 0; ht < 4; ht++)
+#pragma unroll
+        for (int ct = 0; ct < CT; ct++) {
+            const int col = ct * 16 + (lane & 15);
+#pragma unroll
+            for (int j = 0; j < 4; j++) {
+                const float v = acc[ht][ct][j];
+                if (v != 0.0f) atomicAdd(&gW[(ht * 16 + kq + j) * K + col], v);
+            }
+        }
+}
+
+// ---------------------------------------------------------------------------------------
+// Aggregation, embedding, head and bookkeeping kernels: one wave per node row, lane = dim.
+
+__global__ void k_embed(const int32_t* __restrict__ l_off, const int32_t* __restrict__ c_off, int L, int C,
+                        const float* __restrict__ th, float* __restrict__ Hl, float* __restrict__ Hc)
+{
+    const int64_t total = (int64_t)(L + C) * HID;
+    for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
+        const int d = (int)(i & 63);
+        const int64_t r = i >> 6;
+        if (r < L) {
+            const float occ = (float)(l_off[r + 1] - l_off[r]);
+            Hl[i] = occ * 0.125f * th[O_UL + d] + th[O_VL + d];
+        } else {
+            const int64_t c = r - L;
+            const float len = (float)(c_off[c + 1] - c_off[c]);
+            Hc[c * HID + d] = len * 0.25f * th[O_UC + d] + th[O_VC + d];
+        }
+    }
+}
+
+__device__ __forceinline__ int wave_global() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); }
+__device__ __forceinline__ int wave_count() { return (int)((gridDim.x * blockDim.x) >> 6); }
+
+// Xc = bf16([Hc | 1/4 sum_{l in c} Hl])
+__global__ void k_gather_clause(const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_edges, int C,
+                                const float* __restrict__ Hl, const float* __restrict__ Hc, bf16_t* __restrict__ Xc)
+{
+    const int d = threadIdx.x & 63;
+    for (int c = wave_global(); c < C; c += wave_count()) {
+        float s = 0.f;
+        for (int e = c_off[c]; e < c_off[c + 1]; e++) s += Hl[(size_t)c_edges[e] * HID + d];
+        Xc[(size_t)c * 128 + d] = to_bf16(Hc[(size_t)c * HID + d]);
+        Xc[(size_t)c * 128 + 64 + d] = to_bf16(0.25f * s);
+    }
+}
+
+// Xl = bf16([Hl | 1/8 sum_{c ni l} Hc' | H_{l^1}])
+__global__ void k_gather_lit(const int32_t* __restrict__ l_off, const int32_t* __restrict__ l_edges, int L,
+                             const float* __restrict__ Hl, const float* __restrict__ Hc1, bf16_t* __restrict__ Xl)
+{
+    const int d = threadIdx.x & 63;
+    for (int l = wave_global(); l < L; l += wave_count()) {
+        float s = 0.f;
+        for (int e = l_off[l]; e < l_off[l + 1]; e++) s += Hc1[(size_t)l_edges[e] * HID + d];
+        Xl[(size_t)l * 192 + d] = to_bf16(Hl[(size_t)l * HID + d]);
+        Xl[(size_t)l * 192 + 64 + d] = to_bf16(0.125f * s);
+        Xl[(size_t)l * 192 + 128 + d] = to_bf16(Hl[(size_t)(l ^ 1) * HID + d]);
+    }
+}
+
+// Z = bf16([H_{2v+1} | H_{2v}])
+__global__ void k_build_z(int V, const float* __restrict__ Hl, bf16_t* __restrict__ Z)
+{
+    const int d = threadIdx.x & 63;
+    for (int v = wave_global(); v < V; v += wave_count()) {
+        Z[(size_t)v * 128 + d] = to_bf16(Hl[(size_t)(2 * v + 1) * HID + d]);
+        Z[(size_t)v * 128 + 64 + d] = to_bf16(Hl[(size_t)(2 * v) * HID + d]);
+    }
+}
+
+// Y = R W_y^T + b_y (W_y in BF16); with labels also the loss and dY.
+__global__ void k_head(int V, const float* __restrict__ Rv, const bf16_t* __restrict__ Wyq, const float* __restrict__ th,
+                       float* __restrict__ Y, const float* __restrict__ target, const float* __restrict__ phase,
+                       const float* __restrict__ mask, float alpha, float* __restrict__ dY, float* __restrict__ loss)
+{
+    const int d = threadIdx.x & 63;
+    float lsum = 0.f;
+    for (int v = wave_global(); v < V; v += wave_count()) {
+        const float r = Rv[(size_t)v * HID + d];
+        float y0 = r * from_bf16(Wyq[d]);
+        float y1 = r * from_bf16(Wyq[64 + d]);
+        for (int s = 32; s > 0; s >>= 1) {
+            y0 += __shfl_xor(y0, s);
+            y1 += __shfl_xor(y1, s);
+        }
+        y0 += th[O_BY];
+        y1 += th[O_BY + 1];
+        if (d == 0) {
+            Y[2 * (size_t)v] = y0;
+            Y[2 * (size_t)v + 1] = y1;
+            if (target) {
+                const float t = target[v], p = phase[v], mk = mask[v];
+                const float sig = 1.0f / (1.0f + expf(-y1));
+                const float bce = fmaxf(y1, 0.f) - y1 * p + log1pf(expf(-fabsf(y1)));
+                lsum += ((y0 - t) * (y0 - t) + alpha * mk * bce) / (float)V;
+                dY[2 * (size_t)v] = 2.0f * (y0 - t) / (float)V;
+                dY[2 * (size_t)v + 1] = alpha * mk * (sig - p) / (float)V;
+            }
+        }
+    }
+    if (target && d == 0 && lsum != 0.f) atomicAdd(loss, lsum);
+}
+
+// dPo = (dY W_y) * [R > 0] -> BF16, plus gradients of W_y, b_y, b_o.
+__global__ void k_head_bwd(int V, const float* __restrict__ Rv, const float* __restrict__ dY,
+                           const bf16_t* __restrict__ Wyq, bf16_t* __restrict__ dPo, float* __restrict__ grad)
+{
+    const int d = threadIdx.x & 63;
+    const float w0 = from_bf16(Wyq[d]), w1 = from_bf16(Wyq[64 + d]);
+    float gw0 = 0.f, gw1 = 0.f, gb = 0.f, gy0 = 0.f, gy1 = 0.f;
+    for (int v = wave_global(); v < V; v += wave_count()) {
+        const float r = Rv[(size_t)v * HID + d];
+        const float a = dY[2 * (size_t)v], b = dY[2 * (size_t)v + 1];
+        gw0 += a * r;
+        gw1 += b * r;
+        const float dp = r > 0.f ? a * w0 + b * w1 : 0.f;
+        gb += dp;
+        dPo[(size_t)v * HID + d] = to_bf16(dp);
+        if (d == 0) {
+            gy0 += a;
+            gy1 += b;
+        }
+    }
+    atomicAdd(&grad[O_WY + d], gw0);
+    atomicAdd(&grad[O_WY + 64 + d], gw1);
+    atomicAdd(&grad[O_BO + d], gb);
+    if (d == 0) {
+        atomicAdd(&grad[O_BY], gy0);
+        atomicAdd(&grad[O_BY + 1], gy1);
+    }
+}
+
+// dH_l^R from dZ
+__global__ void k_bwd_z(int V, const float* __restrict__ dZ, float* __restrict__ dHl)
+{
+    const int d = threadIdx.x & 63;
+    for (int v = wave_global(); v < V; v += wave_count()) {
+        dHl[(size_t)(2 * v + 1) * HID + d] = dZ[(size_t)v * 128 + d];
+        dHl[(size_t)(2 * v) * HID + d] = dZ[(size_t)v * 128 + 64 + d];
+    }
+}
+
+// dP = dH * [H > 0] -> BF16, and bias gradient.
+__global__ void k_bwd_mask(int N, const float* __restrict__ dH, const float* __restrict__ H, bf16_t* __restrict__ dP,
+                           float* __restrict__ gbias)
+{
+    const int d = threadIdx.x & 63;
+    float gb = 0.f;
+    for (int r = wave_global(); r < N; r += wave_count()) {
+        const size_t i = (size_t)r * HID + d;
+        const float g = H[i] > 0.f ? dH[i] : 0.f;
+        gb += g;
+        dP[i] = to_bf16(g);
+    }
+    atomicAdd(&gbias[d], gb);
+}
+
+// dH_c' = self part (from the next round, stride sstride) + 1/8 sum_{l in c} dXl[l][64 + d];
+// then masked by [H_c' > 0] into dPc (BF16) with the bias gradient.
+__global__ void k_bwd_clause(const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_edges, int C,
+                             const float* __restrict__ self, int sstride, const float* __restrict__ dXl,
+                             const float* __restrict__ Hc1, bf16_t* __restrict__ dPc, float* __restrict__ gbias)
+{
+    const int d = threadIdx.x & 63;
+    float gb = 0.f;
+    for (int c = wave_global(); c < C; c += wave_count()) {
+        float s = self ? self[(size_t)c * sstride + d] : 0.f;
+        float m = 0.f;
+        for (int e = c_off[c]; e < c_off[c + 1]; e++) m += dXl[(size_t)c_edges[e] * 192 + 64 + d];
+        s += 0.125f * m;
+        const float g = Hc1[(size_t)c * HID + d] > 0.f ? s : 0.f;
+        gb += g;
+        dPc[(size_t)c * HID + d] = to_bf16(g);
+    }
+    atomicAdd(&gbias[d], gb);
+}
+
+// dH_l = dXl[l][d] + dXl[l^1][128 + d] + 1/4 sum_{c ni l} dXc[c][64 + d]
+__global__ void k_bwd_lit(const int32_t* __restrict__ l_off, const int32_t* __restrict__ l_edges, int L,
+                          const float* __restrict__ dXl, const float* __restrict__ dXc, float* __restrict__ dHl)
+{
+    const int d = threadIdx.x & 63;
+    for (int l = wave_global(); l < L; l += wave_count()) {
+        float m = 0.f;
+        for (int e = l_off[l]; e < l_off[l + 1]; e++) m += dXc[(size_t)l_edges[e] * 128 + 64 + d];
+        dHl[(size_t)l * HID + d] = dXl[(size_t)l * 192 + d] + dXl[(size_t)(l ^ 1) * 192 + 128 + d] + 0.25f * m;
+    }
+}
+
+// gradients of the embeddings u, v from dH^0 (self part of dXc for clauses, stride 128)
+__global__ void k_bwd_embed(const int32_t* __restrict__ l_off, const int32_t* __restrict__ c_off, int L, int C,
+                            const float* __restrict__ dHl, const float* __restrict__ dXc, float* __restrict__ grad)
+{
+    const int d = threadIdx.x & 63;
+    float gu = 0.f, gv = 0.f, cu = 0.f, cv = 0.f;
+    for (int r = wave_global(); r < L + C; r += wave_count()) {
+        if (r < L) {
+            const float g = dHl[(size_t)r * HID + d];
+            gu += (float)(l_off[r + 1] - l_off[r]) * 0.125f * g;
+            gv += g;
+        } else {
+            const int c = r - L;
+            const float g = dXc[(size_t)c * 128 + d];
+            cu += (float)(c_off[c + 1] - c_off[c]) * 0.25f * g;
+            cv += g;
+        }
+    }
+    atomicAdd(&grad[O_UL + d], gu);
+    atomicAdd(&grad[O_VL + d], gv);
+    atomicAdd(&grad[O_UC + d], cu);
+    atomicAdd(&grad[O_VC + d], cv);
+}
+
+__global__ void k_quantize(const float* __restrict__ th, bf16_t* __restrict__ Wc, bf16_t* __restrict__ Wl,
+                           bf16_t* __restrict__ Wo, bf16_t* __restrict__ Wy)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < 64 * 192; i += gridDim.x * blockDim.x) {
+        if (i < 64 * 128) Wc[i] = to_bf16(th[O_WC + i]);
+        Wl[i] = to_bf16(th[O_WL + i]);
+        if (i < 64 * 128) Wo[i] = to_bf16(th[O_WO + i]);
+        if (i < 128) Wy[i] = to_bf16(th[O_WY + i]);
+    }
+}
+
+__global__ void k_adam(float* __restrict__ th, const float* __restrict__ g, float* __restrict__ m, float* __restrict__ v,
+                       int n, float lr, float b1, float b2, float eps, float c1, float c2)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        const float gi = g[i];
+        const float mi = b1 * m[i] + (1.f - b1) * gi;
+        const float vi = b2 * v[i] + (1.f - b2) * gi * gi;
+        m[i] = mi;
+        v[i] = vi;
+        th[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+}  // namespace gnn
+}  // namespace gpsat
+
+// ---------------------------------------------------------------------------------------
+// Host orchestration and C ABI.
+using namespace gpsat;
+using namespace gpsat::gnn;
+
+struct gpsat_gnn_model {
+    float* th = nullptr;
+    float* grad = nullptr;
+    float* m = nullptr;
+    float* v = nullptr;
+    bf16_t *Wc = nullptr, *Wl = nullptr, *Wo = nullptr, *Wy = nullptr;
+    int64_t step = 0;
+    bool dirty = true;
+};
+
+struct gpsat_gnn_graph {
+    int32_t V = 0, L = 0, C = 0, R = 3;
+    int64_t E = 0;
+    int32_t *c_off = nullptr, *c_edges = nullptr, *l_off = nullptr, *l_edges = nullptr;
+    std::vector<float*> Hl, Hc;
+    std::vector<bf16_t*> Xc, Xl;
+    bf16_t *Z = nullptr, *dP = nullptr;
+    float *Rv = nullptr, *Y = nullptr, *dY = nullptr, *dXa = nullptr, *dXb = nullptr, *dHl = nullptr, *loss = nullptr;
+    float *target = nullptr, *phase = nullptr, *mask = nullptr;
+    hipStream_t st = nullptr;
+    std::vector<void*> owned;
+    ~gpsat_gnn_graph()
+    {
+        if (st) (void)hipStreamSynchronize(st);
+        for (void* p : owned) (void)hipFree(p);
+        if (st) (void)hipStreamDestroy(st);
+    }
+    template <class T>
+    hipError_t alloc(T*& p, size_t n)
+    {
+        void* q = nullptr;
+        hipError_t e = hipMalloc(&q, std::max<size_t>(n, 1) * sizeof(T));
+        if (e == hipSuccess) owned.push_back(q);
+        p = (T*)q;
+        return e;
+    }
+};
+
+#define GNN_TRY(expr)                                                              \
+    do {                                                                           \
+        hipError_t _e = (expr);                                                    \
+        if (_e != hipSuccess) {                                                    \
+            set_error(std::string(#expr) + ": " + hipGetErrorString(_e));          \
+            return _e == hipErrorOutOfMemory ? GPSAT_ERR_NOMEM : GPSAT_ERR_HIP;    \
+        }                                                                          \
+    } while (0)
+
+namespace {
+
+int grid_rows(int64_t rows, int waves_per_block = 4)
+{
+    const int64_t b = (rows + waves_per_block - 1) / waves_per_block;
+    return (int)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
+}
+
+hipError_t forward_pass(gpsat_gnn_model* m, gpsat_gnn_graph* g)
+{
+    hipStream_t st = g->st;
+    if (m->dirty) {
+        hipLaunchKernelGGL(k_quantize, dim3(48), dim3(256), 0, st, m->th, m->Wc, m->Wl, m->Wo, m->Wy);
+        m->dirty = false;
+    }
+    const int64_t rows = (int64_t)(g->L + g->C) * HID;
+    hipLaunchKernelGGL(k_embed, dim3((int)std::min<int64_t>((rows + 255) / 256, 65536)), dim3(256), 0, st, g->l_off,
+                       g->c_off, g->L, g->C, m->th, g->Hl[0], g->Hc[0]);
+    for (int t = 0; t < g->R; t++) {
+        hipLaunchKernelGGL(k_gather_clause, dim3(grid_rows(g->C)), dim3(256), 0, st, g->c_off, g->c_edges, g->C,
+                           g->Hl[t], g->Hc[t], g->Xc[t]);
+        hipLaunchKernelGGL((k_dense_fwd<128>), dim3((g->C + 63) / 64), dim3(256), 0, st, g->Xc[t], m->Wc,
+                           m->th + O_BC, g->Hc[t + 1], g->C);
+        hipLaunchKernelGGL(k_gather_lit, dim3(grid_rows(g->L)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, g->Hl[t],
+                           g->Hc[t + 1], g->Xl[t]);
+        hipLaunchKernelGGL((k_dense_fwd<192>), dim3((g->L + 63) / 64), dim3(256), 0, st, g->Xl[t], m->Wl,
+                           m->th + O_BL, g->Hl[t + 1], g->L);
+    }
+    hipLaunchKernelGGL(k_build_z, dim3(grid_rows(g->V)), dim3(256), 0, st, g->V, g->Hl[g->R], g->Z);
+    hipLaunchKernelGGL((k_dense_fwd<128>), dim3((g->V + 63) / 64), dim3(256), 0, st, g->Z, m->Wo, m->th + O_BO,
+                       g->Rv, g->V);
+    return hipGetLastError();
+}
+
+hipError_t head(gpsat_gnn_model* m, gpsat_gnn_graph* g, bool labels, float alpha)
+{
+    hipLaunchKernelGGL(k_head, dim3(grid_rows(g->V)), dim3(256), 0, g->st, g->V, g->Rv, m->Wy, m->th, g->Y,
+                       labels ? g->target : nullptr, g->phase, g->mask, alpha, g->dY, g->loss);
+    return hipGetLastError();
+}
+
+hipError_t backward_pass(gpsat_gnn_model* m, gpsat_gnn_graph* g)
+{
+    hipStream_t st = g->st;
+    const int wblk = 256;
+    // head and readout
+    hipLaunchKernelGGL(k_head_bwd, dim3(grid_rows(g->V)), dim3(256), 0, st, g->V, g->Rv, g->dY, m->Wy, g->dP, m->grad);
+    hipLaunchKernelGGL((k_dense_bwd_dw<128>), dim3(std::min(wblk, (g->V + 63) / 64)), dim3(64), 0, st, g->dP, g->Z,
+                       m->grad + O_WO, g->V);
+    hipLaunchKernelGGL((k_dense_bwd_dx<128>), dim3((g->V + 63) / 64), dim3(256), 0, st, g->dP, m->Wo, g->dXa, g->V);
+    hipLaunchKernelGGL(k_bwd_z, dim3(grid_rows(g->V)), dim3(256), 0, st, g->V, g->dXa, g->dHl);
+    // dXa holds dXl (L x 192), dXb holds dXc (C x 128)
+    const float* self = nullptr;
+    for (int t = g->R - 1; t >= 0; t--) {
+        hipLaunchKernelGGL(k_bwd_mask, dim3(grid_rows(g->L)), dim3(256), 0, st, g->L, g->dHl, g->Hl[t + 1], g->dP,
+                           m->grad + O_BL);
+        hipLaunchKernelGGL((k_dense_bwd_dw<192>), dim3(std::min(wblk, (g->L + 63) / 64)), dim3(64), 0, st, g->dP,
+                           g->Xl[t], m->grad + O_WL, g->L);
+        hipLaunchKernelGGL((k_dense_bwd_dx<192>), dim3((g->L + 63) / 64), dim3(256), 0, st, g->dP, m->Wl, g->dXa, g->L);
+        // the self part of dH_c^{t+1} comes from round t+1's clause update (dXc of that round)
+        hipLaunchKernelGGL(k_bwd_clause, dim3(grid_rows(g->C)), dim3(256), 0, st, g->c_off, g->c_edges, g->C, self, 128,
+                           g->dXa, g->Hc[t + 1], g->dP, m->grad + O_BC);
+        hipLaunchKernelGGL((k_dense_bwd_dw<128>), dim3(std::min(wblk, (g->C + 63) / 64)), dim3(64), 0, st, g->dP,
+                           g->Xc[t], m->grad + O_WC, g->C);
+        hipLaunchKernelGGL((k_dense_bwd_dx<128>), dim3((g->C + 63) / 64), dim3(256), 0, st, g->dP, m->Wc, g->dXb, g->C);
+        hipLaunchKernelGGL(k_bwd_lit, dim3(grid_rows(g->L)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, g->dXa,
+                           g->dXb, g->dHl);
+        self = g->dXb;
+    }
+    hipLaunchKernelGGL(k_bwd_embed, dim3(grid_rows(g->L + g->C)), dim3(256), 0, st, g->l_off, g->c_off, g->L, g->C,
+                       g->dHl, g->dXb, m->grad);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+extern "C" {
+
+int gpsat_gnn_param_count(int32_t* n)
+{
+    if (!n) {
+        set_error("null argument");
+        return GPSAT_ERR_ARG;
+    }
+    *n = N_PARAMS;
+    return GPSAT_OK;
+}
+
+int gpsat_gnn_model_create(const float* theta, gpsat_gnn_model** out)
+{
+    if (!out || !theta) {
+        set_error("null argument");
+        return GPSAT_ERR_ARG;
+    }
+    *out = nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) {
+        set_error("no HIP device visible");
+        return GPSAT_ERR_NODEVICE;
+    }
+    gpsat_gnn_model* m = new gpsat_gnn_model();
+    hipError_t e = hipMalloc(&m->th, 4 * N_PARAMS);
+    if (e == hipSuccess) e = hipMalloc(&m->grad, 4 * N_PARAMS);
+    if (e == hipSuccess) e = hipMalloc(&m->m, 4 * N_PARAMS);
+    if (e == hipSuccess) e = hipMalloc(&m->v, 4 * N_PARAMS);
+    if (e == hipSuccess) e = hipMalloc(&m->Wc, 2 * 64 * 128);
+    if (e == hipSuccess) e = hipMalloc(&m->Wl, 2 * 64 * 192);
+    if (e == hipSuccess) e = hipMalloc(&m->Wo, 2 * 64 * 128);
+    if (e == hipSuccess) e = hipMalloc(&m->Wy, 2 * 128);
+    if (e == hipSuccess) e = hipMemcpy(m->th, theta, 4 * N_PARAMS, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemset(m->m, 0, 4 * N_PARAMS);
+    if (e == hipSuccess) e = hipMemset(m->v, 0, 4 * N_PARAMS);
+    if (e != hipSuccess) {
+        set_error(std::string("gnn model: ") + hipGetErrorString(e));
+        gpsat_gnn_model_destroy(m);
+        return GPSAT_ERR_HIP;
+    }
+    *out = m;
+    return GPSAT_OK;
+}
+
+int gpsat_gnn_model_destroy(gpsat_gnn_model* m)
+{
+    if (!m) return GPSAT_OK;
+    (void)hipDeviceSynchronize();
+    for (void* p : {(void*)m->th, (void*)m->grad, (void*)m->m, (void*)m->v, (void*)m->Wc, (void*)m->Wl, (void*)m->Wo,
+                    (void*)m->Wy})
+        (void)hipFree(p);
+    delete m;
+    return GPSAT_OK;
+}
+
+int gpsat_gnn_model_params(gpsat_gnn_model* m, float* theta)
+{
+    if (!m || !theta) {
+        set_error("null argument");
+        return GPSAT_ERR_ARG;
+    }
+    GNN_TRY(hipDeviceSynchronize());
+    GNN_TRY(hipMemcpy(theta, m->th, 4 * N_PARAMS, hipMemcpyDeviceToHost));
+    return GPSAT_OK;
+}
+
+int gpsat_gnn_graph_create(gpsat_batch* b, int32_t rounds, gpsat_gnn_graph** out)
+{
+    if (!b || !out || rounds < 1 || rounds > 16) {
+        set_error("bad argument");
+        return GPSAT_ERR_ARG;
+    }
+    *out = nullptr;
+    DeviceBatch& d = b->d;
+    gpsat_gnn_graph* g = new gpsat_gnn_graph();
+    g->R = rounds;
+    // CSR both ways over global node ids
+    std::vector<int64_t> var_base(d.n_inst + 1, 0);
+    for (int32_t i = 0; i < d.n_inst; i++) var_base[i + 1] = var_base[i] + d.h_meta[i].n;
+    g->V = (int32_t)var_base[d.n_inst];
+    g->L = 2 * g->V;
+    g->C = (int32_t)d.n_clauses;
+    g->E = d.n_lits;
+    std::vector<int32_t> c_off(g->C + 1, 0), c_edges(g->E), l_off(g->L + 1, 0), l_edges(g->E);
+    int32_t cg = 0;
+    for (int32_t i = 0; i < d.n_inst; i++) {
+        const InstMeta& mi = d.h_meta[i];
+        const int32_t* co = d.h_coff.data() + mi.coff_base;
+        for (int32_t c = 0; c < mi.m; c++, cg++) {
+            c_off[cg + 1] = (int32_t)(mi.lit_base + co[c + 1]);
+            for (int32_t k = co[c]; k < co[c + 1]; k++) {
+                const int32_t ln = (int32_t)(2 * var_base[i] + d.h_lits[mi.lit_base + k]);
+                c_edges[mi.lit_base + k] = ln;
+                l_off[ln + 1]++;
+            }
+        }
+    }
+    for (int32_t l = 0; l < g->L; l++) l_off[l + 1] += l_off[l];
+    std::vector<int32_t> fill(l_off.begin(), l_off.end() - 1);
+    for (int32_t c = 0; c < g->C; c++)
+        for (int32_t e = c_off[c]; e < c_off[c + 1]; e++) l_edges[fill[c_edges[e]]++] = c;
+    hipError_t e = hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking);
+    const size_t L = g->L, C = g->C, V = g->V;
+    g->Hl.resize(rounds + 1);
+    g->Hc.resize(rounds + 1);
+    g->Xc.resize(rounds);
+    g->Xl.resize(rounds);
+    if (e == hipSuccess) e = g->alloc(g->c_off, C + 1);
+    if (e == hipSuccess) e = g->alloc(g->c_edges, g->E);
+    if (e == hipSuccess) e = g->alloc(g->l_off, L + 1);
+    if (e == hipSuccess) e = g->alloc(g->l_edges, g->E);
+    for (int t = 0; t <= rounds && e == hipSuccess; t++) {
+        e = g->alloc(g->Hl[t], L * HID);
+        if (e == hipSuccess) e = g->alloc(g->Hc[t], C * HID);
+        if (e == hipSuccess && t < rounds) e = g->alloc(g->Xc[t], C * 128);
+        if (e == hipSuccess && t < rounds) e = g->alloc(g->Xl[t], L * 192);
+    }
+    const size_t maxn = std::max({L, C, V});
+    if (e == hipSuccess) e = g->alloc(g->Z, V * 128);
+    if (e == hipSuccess) e = g->alloc(g->dP, maxn * HID);
+    if (e == hipSuccess) e = g->alloc(g->Rv, V * HID);
+    if (e == hipSuccess) e = g->alloc(g->Y, V * 2);
+    if (e == hipSuccess) e = g->alloc(g->dY, V * 2);
+    if (e == hipSuccess) e = g->alloc(g->dXa, std::max(L * 192, V * 128));
+    if (e == hipSuccess) e = g->alloc(g->dXb, C * 128);
+    if (e == hipSuccess) e = g->alloc(g->dHl, L * HID);
+    if (e == hipSuccess) e = g->alloc(g->loss, 1);
+    if (e == hipSuccess) e = g->alloc(g->target, V);
+    if (e == hipSuccess) e = g->alloc(g->phase, V);
+    if (e == hipSuccess) e = g->alloc(g->mask, V);
+    if (e == hipSuccess) e = hipMemcpy(g->c_off, c_off.data(), 4 * (C + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && g->E) e = hipMemcpy(g->c_edges, c_edges.data(), 4 * g->E, hipMemcpyHostToDevice);
+    if (e == hipSuccess) e = hipMemcpy(g->l_off, l_off.data(), 4 * (L + 1), hipMemcpyHostToDevice);
+    if (e == hipSuccess && g->E) e = hipMemcpy(g->l_edges, l_edges.data(), 4 * g->E, hipMemcpyHostToDevice);
+    if (e != hipSuccess) {
+        set_error(std::string("gnn graph: ") + hipGetErrorString(e));
+        delete g;
+        return e == hipErrorOutOfMemory ? GPSAT_ERR_NOMEM : GPSAT_ERR_HIP;
+    }
+    *out = g;
+    return GPSAT_OK;
+}
+
+int gpsat_gnn_graph_destroy(gpsat_gnn_graph* g)
+{
+    delete g;
+    return GPSAT_OK;
+}
+
+int gpsat_gnn_graph_labels(gpsat_gnn_graph* g, const float* target, const float* phase, const float* mask)
+{
+    if (!g || !target || !phase || !mask) {
+        set_error("null argument");
+        return GPSAT_ERR_ARG;
+    }
+    GNN_TRY(hipMemcpyAsync(g->target, target, 4 * (size_t)g->V, hipMemcpyHostToDevice, g->st));
+    GNN_TRY(hipMemcpyAsync(g->phase, phase, 4 * (size_t)g->V, hipMemcpyHostToDevice, g->st));
+    GNN_TRY(hipMemcpyAsync(g->mask, mask, 4 * (size_t)g->V, hipMemcpyHostToDevice, g->st));
+    GNN_TRY(hipStreamSynchronize(g->st));
+    return GPSAT_OK;
+}
+
+int gpsat_gnn_forward(gpsat_gnn_model* m, gpsat_gnn_graph* g, float* y)
+{
+    if (!m || !g || !y) {
+        set_error("null argument");
+        return GPSAT_ERR_ARG;
+    }
+    GNN_TRY(forward_pass(m, g));
+    GNN_TRY(head(m, g, false, 0.f));
+    GNN_TRY(hipMemcpyAsync(y, g->Y, 8 * (size_t)g->V, hipMemcpyDeviceToHost, g->st));
+    GNN_TRY(hipStreamSynchronize(g->st));
+    return GPSAT_OK;
+}
+
+int gpsat_gnn_loss_grad(gpsat_gnn_model* m, gpsat_gnn_graph* g, float alpha, float* loss, float* grad, float* y)
+{
+    if (!m || !g) {
+        set_error("null argument");
+        return GPSAT_ERR_ARG;
+    }
+    GNN_TRY(hipMemsetAsync(g->loss, 0, 4, g->st));
+    GNN_TRY(hipMemsetAsync(m->grad, 0, 4 * N_PARAMS, g->st));
+    GNN_TRY(forward_pass(m, g));
+    GNN_TRY(head(m, g, true, alpha));
+    GNN_TRY(backward_pass(m, g));
+    if (loss) GNN_TRY(hipMemcpyAsync(loss, g->loss, 4, hipMemcpyDeviceToHost, g->st));
+    if (grad) GNN_TRY(hipMemcpyAsync(grad, m->grad, 4 * N_PARAMS, hipMemcpyDeviceToHost, g->st));
+    if (y) GNN_TRY(hipMemcpyAsync(y, g->Y, 8 * (size_t)g->V, hipMemcpyDeviceToHost, g->st));
+    GNN_TRY(hipStreamSynchronize(g->st));
+    return GPSAT_OK;
+}
+
+int gpsat_gnn_train_step(gpsat_gnn_model* m, gpsat_gnn_graph* g, float alpha, float lr, float* loss)
+{
+    if (!m || !g || !(lr > 0.f)) {
+        set_error("bad argument");
+        return GPSAT_ERR_ARG;
+    }
+    GNN_TRY(hipMemsetAsync(g->loss, 0, 4, g->st));
+    GNN_TRY(hipMemsetAsync(m->grad, 0, 4 * N_PARAMS, g->st));
+    GNN_TRY(forward_pass(m, g));
+    GNN_TRY(head(m, g, true, alpha));
+    GNN_TRY(backward_pass(m, g));
+    m->step++;
+    const float b1 = 0.9f, b2 = 0.999f;
+    const float c1 = 1.f - powf(b1, (float)m->step), c2 = 1.f - powf(b2, (float)m->step);
+    hipLaunchKernelGGL(k_adam, dim3(64), dim3(256), 0, g->st, m->th, m->grad, m->m, m->v, N_PARAMS, lr, b1, b2, 1e-8f,
+                       c1, c2);
+    m->dirty = true;
+    GNN_TRY(hipGetLastError());
+    if (loss) GNN_TRY(hipMemcpyAsync(loss, g->loss, 4, hipMemcpyDeviceToHost, g->st));
+    GNN_TRY(hipStreamSynchronize(g->st));
+    return GPSAT_OK;
+}
+
+}  // extern "C"

@@ -180,6 +180,28 @@ int gpsat_solve(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t *job_in
                 int32_t *status, uint32_t *model_rows, int32_t *verified, int64_t *job_stats, int32_t *var_bumps,
                 gpsat_solve_info *info);
 
+/* ---- a17: the learned branching network (no reference counterpart).  Message passing
+ * over the literal-clause incidence graph of a batch: 64 hidden units, `rounds` rounds,
+ * BF16 weights with FP32 accumulation on MFMA; per var y = (branching score, phase
+ * logit).  A model holds FP32 master parameters (gpsat_gnn_param_count floats) and
+ * Adam state; a graph holds one batch's incidence structure, activations and labels.
+ * Labels: target score, phase (0/1) and phase mask per var, instances back to back. */
+typedef struct gpsat_gnn_model gpsat_gnn_model;
+typedef struct gpsat_gnn_graph gpsat_gnn_graph;
+int gpsat_gnn_param_count(int32_t *n);
+int gpsat_gnn_model_create(const float *theta, gpsat_gnn_model **out);
+int gpsat_gnn_model_destroy(gpsat_gnn_model *m);
+int gpsat_gnn_model_params(gpsat_gnn_model *m, float *theta);
+int gpsat_gnn_graph_create(gpsat_batch *b, int32_t rounds, gpsat_gnn_graph **out);
+int gpsat_gnn_graph_destroy(gpsat_gnn_graph *g);
+int gpsat_gnn_graph_labels(gpsat_gnn_graph *g, const float *target, const float *phase, const float *mask);
+/* y[V*2] */
+int gpsat_gnn_forward(gpsat_gnn_model *m, gpsat_gnn_graph *g, float *y);
+/* mean loss over vars and its gradient (no update); loss, grad and y may be NULL */
+int gpsat_gnn_loss_grad(gpsat_gnn_model *m, gpsat_gnn_graph *g, float alpha, float *loss, float *grad, float *y);
+/* one Adam step (beta 0.9 / 0.999) on the batch */
+int gpsat_gnn_train_step(gpsat_gnn_model *m, gpsat_gnn_graph *g, float alpha, float lr, float *loss);
+
 #ifdef __cplusplus
 }
 #endif
