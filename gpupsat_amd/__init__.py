@@ -235,7 +235,7 @@ class SolveInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-JOB_STAT_FIELDS = ("status", "conflicts", "decisions", "props", "learned", "restarts", "max_level", "reserved")
+JOB_STAT_FIELDS = ("status", "conflicts", "decisions", "props", "learned", "restarts", "max_level", "time_ns")
 
 
 class GpuBatch:

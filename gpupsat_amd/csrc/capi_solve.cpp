@@ -277,6 +277,10 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
         HIP_OR(hipMemcpy(model_rows, d_models, sizeof(uint32_t) * d.total_words, hipMemcpyDeviceToHost));
     std::vector<int64_t> js((size_t)std::max(n_jobs, 1) * JS_COUNT);
     HIP_OR(hipMemcpy(js.data(), d_jstats, sizeof(int64_t) * js.size(), hipMemcpyDeviceToHost));
+    for (int32_t j = 0; j < n_jobs; j++) {   // device clock ticks -> nanoseconds
+        int64_t& t = js[(size_t)j * JS_COUNT + JS_TICKS];
+        t = (int64_t)((double)t * 1e6 / (double)clock_khz);
+    }
     if (job_stats) memcpy(job_stats, js.data(), sizeof(int64_t) * (size_t)n_jobs * JS_COUNT);
     if (var_bumps) HIP_OR(hipMemcpy(var_bumps, d_bumps, sizeof(int32_t) * var_base[N], hipMemcpyDeviceToHost));
 

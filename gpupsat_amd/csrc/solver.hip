@@ -763,7 +763,9 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a)
         j = uni(__shfl(j, 0));
         if (j >= a.n_jobs) break;
         const JobDesc jd = a.jobs[j];
+        const uint64_t t_job = wall_clock64();
         const int32_t st = run_job(J, a, jd);
+        const uint64_t job_ticks = wall_clock64() - t_job;
         __syncthreads();
         const int32_t inst = jd.inst;
         if (st == 0) {
@@ -804,7 +806,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a)
             s[JS_LEARNED] = J.learned;
             s[JS_RESTARTS] = J.restarts;
             s[JS_LEVELMAX] = J.levelmax;
-            s[7] = 0;
+            s[JS_TICKS] = (int64_t)job_ticks;
         }
         __syncthreads();
     }

@@ -30,7 +30,7 @@ struct JobDesc {
     int32_t order;            // position of the job in the caller's job list
 };
 
-enum JobStat { JS_STATUS = 0, JS_CONFLICTS, JS_DECISIONS, JS_PROPS, JS_LEARNED, JS_RESTARTS, JS_LEVELMAX, JS_COUNT = 8 };
+enum JobStat { JS_STATUS = 0, JS_CONFLICTS, JS_DECISIONS, JS_PROPS, JS_LEARNED, JS_RESTARTS, JS_LEVELMAX, JS_TICKS, JS_COUNT };
 
 struct SolveArgs {
     const InstMeta* meta;

@@ -28,20 +28,23 @@ def param_count() -> int:
 
 
 def init_params(seed: int = 0) -> np.ndarray:
-    """He-style initial weights; the same recipe as the oracle's init (different RNG use
-    is fine: parameters are data)."""
+    """Initial parameters for training (normal weights with variance 1/fan_in, small
+    output layer)."""
     rng = np.random.default_rng(seed)
     H = 64
     shapes = [("u_l", (H,)), ("v_l", (H,)), ("u_c", (H,)), ("v_c", (H,)), ("W_c", (H, 2 * H)), ("b_c", (H,)),
               ("W_l", (H, 3 * H)), ("b_l", (H,)), ("W_o", (H, 2 * H)), ("b_o", (H,)), ("W_y", (2, H)), ("b_y", (2,))]
     out = []
     for name, shp in shapes:
-        if name.startswith("W"):
-            out.append(rng.normal(0, np.sqrt(2.0 / shp[1]), size=shp))
+        if name == "W_y":
+            out.append(rng.normal(0, 0.05, size=shp))
+        elif name.startswith("W"):
+            # variance 1/fan_in keeps the summed messages from growing round after round
+            out.append(rng.normal(0, np.sqrt(1.0 / shp[1]), size=shp))
         elif name in ("u_l", "u_c"):
-            out.append(rng.normal(0, 0.5, size=shp))
+            out.append(rng.normal(0, 0.3, size=shp))
         elif name in ("v_l", "v_c"):
-            out.append(np.abs(rng.normal(0, 0.5, size=shp)))
+            out.append(np.abs(rng.normal(0, 0.3, size=shp)))
         else:
             out.append(np.zeros(shp))
     theta = np.concatenate([o.reshape(-1) for o in out]).astype(np.float32)

@@ -173,7 +173,7 @@ int gpsat_solve_opts_default(gpsat_solve_opts *o);
  * status[n_inst] (0 SAT, 1 UNSAT, 2 UNDEF); model_rows (gpsat_batch_row_words(b, 1)
  * words: one literal bitmap per instance, valid where SAT); verified[n_inst] (1 where a
  * SAT model was re-checked on the GPU); job_stats[n_jobs*8] (status, conflicts,
- * decisions, propagations, learned, restarts, max level, 0); var_bumps (per var, how
+ * decisions, propagations, learned, restarts, max level, job time in ns); var_bumps (per var, how
  * often conflict analysis met it).  Every output may be NULL except status. */
 int gpsat_solve(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t *job_inst, const int32_t *cube_off,
                 const int32_t *cube_lits, int32_t n_jobs, const float *act_init, const int8_t *phase_init,
