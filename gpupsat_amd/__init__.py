@@ -209,7 +209,8 @@ class SolveOpts(ctypes.Structure):
         ("learnt_cap", ctypes.c_int32), ("larena_cap", ctypes.c_int32), ("watch_slack", ctypes.c_int32),
         ("max_conflicts", ctypes.c_int64), ("max_decisions", ctypes.c_int64), ("var_decay", ctypes.c_float),
         ("phase_init", ctypes.c_int32), ("max_ms", ctypes.c_double), ("waves", ctypes.c_int32),
-        ("clause_mem", ctypes.c_int32), ("verify", ctypes.c_int32),
+        ("clause_mem", ctypes.c_int32), ("verify", ctypes.c_int32), ("chunk", ctypes.c_int32),
+        ("epoch_conflicts", ctypes.c_int64), ("epoch_growth", ctypes.c_float), ("max_epochs", ctypes.c_int32),
     ]
 
     @staticmethod
@@ -228,7 +229,8 @@ class SolveInfo(ctypes.Structure):
         ("kernel_ms", ctypes.c_double), ("verify_ms", ctypes.c_double), ("waves", ctypes.c_int32),
         ("clause_mem", ctypes.c_int32), ("lds_bytes", ctypes.c_int64), ("workspace_bytes", ctypes.c_int64),
         ("n_sat", ctypes.c_int32), ("n_unsat", ctypes.c_int32), ("n_undef", ctypes.c_int32),
-        ("n_verified", ctypes.c_int32), ("n_errors", ctypes.c_int32),
+        ("n_verified", ctypes.c_int32), ("n_errors", ctypes.c_int32), ("epochs", ctypes.c_int32),
+        ("chunk", ctypes.c_int32),
     ]
 
     def as_dict(self):

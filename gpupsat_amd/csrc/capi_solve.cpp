@@ -70,6 +70,10 @@ int gpsat_solve_opts_default(gpsat_solve_opts* o)
     o->waves = 0;
     o->clause_mem = 0;
     o->verify = 1;
+    o->chunk = 0;
+    o->epoch_conflicts = 0;
+    o->epoch_growth = 2.0f;
+    o->max_epochs = 16;
     return GPSAT_OK;
 }
 
@@ -182,6 +186,10 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
         waves = prop.multiProcessorCount * per_cu;
     }
     waves = std::max(1, std::min(waves, n_jobs));
+    // chunks of consecutive jobs let cubes of one instance reuse its watch lists; a few
+    // chunks per wavefront keep the queue balanced
+    a.chunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(64, (int64_t)n_jobs / ((int64_t)waves * 4)));
+    if (o.chunk > 0) a.chunk = o.chunk;
 
     // ---- device buffers
     DevMem mem;
@@ -245,12 +253,99 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     a.var_bumps = d_bumps;
     a.ws = d_ws;
 
+    const bool epochs = o.epoch_conflicts > 0;
+    int32_t epochs_run = 1;
+    int32_t* h_remaining = nullptr;
+    if (epochs) {
+        if (!(o.epoch_growth >= 1.0f) || o.max_epochs < 1 || o.max_epochs > 64) {
+            set_error("invalid epoch options");
+            return GPSAT_ERR_ARG;
+        }
+        int32_t *d_active, *d_nact, *d_epoch, *d_remaining;
+        int64_t* d_budget;
+        float* d_astore;
+        int8_t* d_pstore;
+        HIP_OR(mem.alloc(d_active, 2 * (size_t)n_jobs));
+        HIP_OR(mem.alloc(d_nact, 2));
+        HIP_OR(mem.alloc(d_epoch, 1));
+        HIP_OR(mem.alloc(d_remaining, 1));
+        HIP_OR(mem.alloc(d_budget, o.max_epochs));
+        HIP_OR(mem.alloc(d_astore, (size_t)var_base[N]));
+        HIP_OR(mem.alloc(d_pstore, (size_t)var_base[N]));
+        std::vector<int32_t> ident(n_jobs);
+        for (int32_t j = 0; j < n_jobs; j++) ident[j] = j;
+        const int32_t nact[2] = {n_jobs, 0}, zero = 0;
+        std::vector<int64_t> budget(o.max_epochs);
+        double bud = (double)o.epoch_conflicts;
+        for (int32_t e = 0; e < o.max_epochs; e++, bud *= o.epoch_growth) budget[e] = (int64_t)std::min(bud, 1e15);
+        std::vector<float> astore(var_base[N], 0.0f);
+        std::vector<int8_t> pstore(var_base[N], (int8_t)(o.phase_init ? 1 : 0));
+        if (act_init) astore.assign(act_init, act_init + var_base[N]);
+        if (phase_init) pstore.assign(phase_init, phase_init + var_base[N]);
+        HIP_OR(hipMemcpyAsync(d_active, ident.data(), 4 * (size_t)n_jobs, hipMemcpyHostToDevice, st));
+        HIP_OR(hipMemcpyAsync(d_nact, nact, 8, hipMemcpyHostToDevice, st));
+        HIP_OR(hipMemcpyAsync(d_epoch, &zero, 4, hipMemcpyHostToDevice, st));
+        HIP_OR(hipMemcpyAsync(d_budget, budget.data(), 8 * budget.size(), hipMemcpyHostToDevice, st));
+        HIP_OR(hipMemcpyAsync(d_astore, astore.data(), 4 * astore.size(), hipMemcpyHostToDevice, st));
+        HIP_OR(hipMemcpyAsync(d_pstore, pstore.data(), pstore.size(), hipMemcpyHostToDevice, st));
+        HIP_OR(hipStreamSynchronize(st));
+        HIP_OR(hipHostMalloc((void**)&h_remaining, sizeof(int32_t), hipHostMallocDefault));
+        a.epoch = d_epoch;
+        a.active = d_active;
+        a.n_active = d_nact;
+        a.epoch_budget = d_budget;
+        a.act_store = d_astore;
+        a.phase_store = d_pstore;
+        a.act_init = d_astore;
+        a.phase_init = d_pstore;
+    }
+
     hipEvent_t e0, e1, e2;
     HIP_OR(hipEventCreate(&e0));
     HIP_OR(hipEventCreate(&e1));
     HIP_OR(hipEventCreate(&e2));
     HIP_OR(hipEventRecord(e0, st));
-    HIP_OR(launch_solve(a, use_lds, waves, lds_bytes, st));
+    if (!epochs) {
+        HIP_OR(launch_solve(a, use_lds, waves, lds_bytes, st));
+    } else {
+        // one restart epoch = one replay of this graph: empty the queue, run the persistent
+        // solve over the open jobs, re-queue the stopped ones, publish how many remain
+        hipGraph_t graph = nullptr;
+        hipGraphExec_t exec = nullptr;
+        int32_t* d_remaining_ptr = nullptr;
+        HIP_OR(hipMalloc(&d_remaining_ptr, sizeof(int32_t)));
+        mem.ptrs.push_back(d_remaining_ptr);
+        HIP_OR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
+        hipError_t ce = launch_epoch_begin(a, st);
+        if (ce == hipSuccess) ce = launch_solve(a, use_lds, waves, lds_bytes, st);
+        if (ce == hipSuccess) ce = launch_epoch_end(a, d_remaining_ptr, st);
+        if (ce == hipSuccess) ce = hipMemcpyAsync(h_remaining, d_remaining_ptr, sizeof(int32_t), hipMemcpyDeviceToHost, st);
+        hipError_t ee = hipStreamEndCapture(st, &graph);
+        if (ce != hipSuccess || ee != hipSuccess) {
+            if (graph) (void)hipGraphDestroy(graph);
+            (void)hipHostFree(h_remaining);
+            return hip_fail("capturing the epoch graph", ce != hipSuccess ? ce : ee);
+        }
+        hipError_t ie = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+        if (ie != hipSuccess) {
+            (void)hipGraphDestroy(graph);
+            (void)hipHostFree(h_remaining);
+            return hip_fail("hipGraphInstantiate", ie);
+        }
+        epochs_run = 0;
+        hipError_t le = hipSuccess;
+        for (int32_t e = 0; e < o.max_epochs; e++) {
+            le = hipGraphLaunch(exec, st);
+            if (le == hipSuccess) le = hipStreamSynchronize(st);
+            if (le != hipSuccess) break;
+            epochs_run++;
+            if (*h_remaining == 0) break;
+        }
+        (void)hipGraphExecDestroy(exec);
+        (void)hipGraphDestroy(graph);
+        (void)hipHostFree(h_remaining);
+        if (le != hipSuccess) return hip_fail("epoch replay", le);
+    }
     HIP_OR(hipEventRecord(e1, st));
     // re-verify every model: clause counts of the model rows (one row per instance)
     int32_t* d_counts = nullptr;
@@ -291,6 +386,8 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     inf.clause_mem = use_lds ? 1 : 2;
     inf.lds_bytes = (int64_t)lds_bytes;
     inf.workspace_bytes = use_lds ? 0 : (int64_t)waves * a.ws_ints_per_wave * 4;
+    inf.epochs = epochs_run;
+    inf.chunk = a.chunk;
     for (int32_t j = 0; j < n_jobs; j++)
         if (js[(size_t)j * JS_COUNT + JS_STATUS] == 4) inf.n_errors++;
     for (int32_t i = 0; i < N; i++) {

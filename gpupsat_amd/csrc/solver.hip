@@ -94,6 +94,9 @@ struct Ctx {
     int32_t nlearnt, larena_used;
     float inc;
     int32_t stamp;
+    int32_t base_trail;      // trail entries of the formula's unit clauses
+    int32_t setup_inst;      // instance whose watch lists are built (-1: none)
+    bool setup_ok;
     int64_t conflicts, decisions, props, learned, restarts;
     int32_t levelmax;
 };
@@ -564,19 +567,20 @@ __device__ void carve_clauses(Ctx& J, int32_t* c, const SolveArgs& a)
     J.larena = c;
 }
 
-// Returns the job verdict: 0 SAT, 1 UNSAT, 2 UNDEF (budget), 3 skipped (instance decided).
-__device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd)
+// Build the per-instance part of a job: watch lists of the original clauses and the
+// formula's unit clauses at level 0.  Returns false if the formula is falsified there.
+__device__ bool setup_instance(Ctx& J, const SolveArgs& a, int32_t inst)
 {
     const int lane = lane_id();
     const SolveOptsDev& o = a.o;
-    const InstMeta mi = a.meta[jd.inst];
-    J.inst = jd.inst;
+    const InstMeta mi = a.meta[inst];
+    J.inst = inst;
     J.n = uni(mi.n);
     J.m = uni(mi.m);
     J.lits = a.lits + mi.lit_base;
     J.coff = a.coff + mi.coff_base;
     const int32_t n = J.n, m = J.m, n2 = 2 * n;
-    const int64_t vb = a.var_base[jd.inst];
+    const int64_t vb = a.var_base[inst];
     J.slack = o.watch_slack;
     J.learnt_cap = o.learnt_cap;
     J.larena_cap = o.larena_cap;
@@ -587,12 +591,6 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd)
     J.larena_used = 0;
     J.inc = 1.0f;
     J.stamp = 0;
-    J.conflicts = J.decisions = J.props = J.learned = J.restarts = 0;
-    J.levelmax = 0;
-
-    if (load_status(&a.inst_status[jd.inst]) != 2) return 3;
-    const uint64_t t_start = wall_clock64();
-
     for (int32_t v = lane; v < n; v += kWave) {
         J.seen[v] = 0;
         J.level[v] = 0;
@@ -666,6 +664,51 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd)
             else if (val == 0) assign(J, u, -1);
         }
     }
+    J.base_trail = J.tsize;
+    __syncthreads();
+    return !unsat;
+}
+
+// Next cube of the instance already set up on this wavefront: undo everything above the
+// formula's unit clauses and forget the learned clauses (they may rest on the previous
+// cube).  Every remaining watch pair is repaired by propagating the units again from the
+// start of the trail.  Activities and saved phases carry over.
+__device__ void reset_instance(Ctx& J)
+{
+    const int lane = lane_id();
+    for (int32_t t = J.base_trail + lane; t < J.tsize; t += kWave) {
+        const int32_t u = J.trail[t];
+        J.lv[u] = 0;
+        J.lv[u ^ 1] = 0;
+        J.reason[u >> 1] = -1;
+    }
+    for (int32_t l = lane; l < 2 * J.n; l += kWave) J.wcnt_l[l] = 0;
+    for (int32_t v = lane; v < J.n; v += kWave) J.bumps[v] = 0;
+    __syncthreads();
+    J.tsize = J.base_trail;
+    J.qhead = 0;
+    J.dl = 0;
+    J.nlearnt = 0;
+    J.larena_used = 0;
+}
+
+// Returns the job verdict: 0 SAT, 1 UNSAT, 2 UNDEF (budget), 3 skipped (instance decided).
+__device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_t max_conflicts)
+{
+    const SolveOptsDev& o = a.o;
+    J.conflicts = J.decisions = J.props = J.learned = J.restarts = 0;
+    J.levelmax = 0;
+    if (load_status(&a.inst_status[jd.inst]) != 2) return 3;
+    const uint64_t t_start = wall_clock64();
+    if (J.setup_inst == jd.inst) {
+        reset_instance(J);
+    } else {
+        J.setup_ok = setup_instance(J, a, jd.inst);
+        J.setup_inst = jd.inst;
+    }
+    if (!J.setup_ok) return 1;
+    const int32_t m = J.m;
+    bool unsat = false;
     // the cube's literals are facts of this job
     for (int32_t k = 0; k < jd.cube_len && !unsat; k++) {
         const int32_t u = uni(a.cube_lits[jd.cube_off + k]);
@@ -716,7 +759,7 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd)
             __syncthreads();
             J.inc *= decay_inv;
             if (J.inc > 1e30f) rescale_activity(J);
-            if (o.max_conflicts > 0 && J.conflicts >= o.max_conflicts) return 2;
+            if (max_conflicts > 0 && J.conflicts >= max_conflicts) return 2;
             if (o.max_ticks > 0 && (J.conflicts & 63) == 0 && (int64_t)(wall_clock64() - t_start) > o.max_ticks) return 2;
         } else {
             if (restart_cnt >= restart_thr) {
@@ -757,14 +800,33 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a)
     int32_t* cpart = kLds ? (int32_t*)(smem + Layout::state_bytes(a.max_n))
                           : a.ws + (int64_t)blockIdx.x * a.ws_ints_per_wave;
     carve_clauses(J, cpart, a);
+    J.setup_inst = -1;
+    J.setup_ok = false;
+    J.n = 0;
+    int32_t n_jobs = a.n_jobs;
+    const int32_t* order = nullptr;
+    int64_t budget = a.o.max_conflicts;
+    if (a.epoch) {
+        const int32_t e = uni(*a.epoch);
+        n_jobs = uni(a.n_active[e & 1]);
+        order = a.active + (int64_t)(e & 1) * a.n_jobs;
+        budget = a.epoch_budget[e];
+    }
+    int32_t j = 0, j_end = 0;
     for (;;) {
-        int32_t j = 0;
-        if (lane == 0) j = atomicAdd(a.queue_head, 1);
-        j = uni(__shfl(j, 0));
-        if (j >= a.n_jobs) break;
-        const JobDesc jd = a.jobs[j];
+        if (j >= j_end) {
+            // grab the next chunk of consecutive jobs (cubes of one instance are adjacent)
+            int32_t k = 0;
+            if (lane == 0) k = atomicAdd(a.queue_head, 1);
+            k = uni(__shfl(k, 0));
+            j = k * a.chunk;
+            if (j >= n_jobs) break;
+            j_end = min(j + a.chunk, n_jobs);
+        }
+        const JobDesc jd = a.jobs[order ? order[j] : j];
+        j++;
         const uint64_t t_job = wall_clock64();
-        const int32_t st = run_job(J, a, jd);
+        const int32_t st = run_job(J, a, jd, budget);
         const uint64_t job_ticks = wall_clock64() - t_job;
         __syncthreads();
         const int32_t inst = jd.inst;
@@ -791,6 +853,14 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a)
             }
         } else if (st == 2) {
             if (lane == 0) atomicAdd(&a.inst_undef[inst], 1);
+            if (a.act_store) {
+                // a job stopped at an epoch boundary restarts next epoch from these
+                const int64_t vb = a.var_base[inst];
+                for (int32_t v = lane; v < J.n; v += kWave) {
+                    a.act_store[vb + v] = J.act[v];
+                    a.phase_store[vb + v] = J.phase[v];
+                }
+            }
         }
         if (a.var_bumps) {
             const int64_t vb = a.var_base[inst];
@@ -799,17 +869,66 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a)
         }
         if (a.job_stats && lane == 0) {
             int64_t* s = a.job_stats + (int64_t)jd.order * JS_COUNT;
+            // counters add up over the epochs a job runs in
             s[JS_STATUS] = st;
-            s[JS_CONFLICTS] = J.conflicts;
-            s[JS_DECISIONS] = J.decisions;
-            s[JS_PROPS] = J.props;
-            s[JS_LEARNED] = J.learned;
-            s[JS_RESTARTS] = J.restarts;
-            s[JS_LEVELMAX] = J.levelmax;
-            s[JS_TICKS] = (int64_t)job_ticks;
+            s[JS_CONFLICTS] += J.conflicts;
+            s[JS_DECISIONS] += J.decisions;
+            s[JS_PROPS] += J.props;
+            s[JS_LEARNED] += J.learned;
+            s[JS_RESTARTS] += J.restarts;
+            s[JS_LEVELMAX] = max(s[JS_LEVELMAX], (int64_t)J.levelmax);
+            s[JS_TICKS] += (int64_t)job_ticks;
         }
         __syncthreads();
     }
+}
+
+// Restart epochs.  begin: empty the queue and the next job list.  end: every job of
+// this epoch stopped by the budget whose instance is still open goes to the next list;
+// then the epoch counter advances and the number of open jobs is published.
+__global__ void k_epoch_begin(int32_t* queue_head, const int32_t* epoch, int32_t* n_active)
+{
+    if (threadIdx.x == 0) {
+        *queue_head = 0;
+        n_active[(*epoch + 1) & 1] = 0;
+    }
+}
+
+__global__ void k_epoch_end(SolveArgs a, int32_t* active, int32_t* n_active)
+{
+    const int32_t e = *a.epoch;
+    const int32_t n = n_active[e & 1];
+    const int32_t* cur = active + (int64_t)(e & 1) * a.n_jobs;
+    int32_t* next = active + (int64_t)((e + 1) & 1) * a.n_jobs;
+    for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
+        const int32_t job = cur[k];
+        const JobDesc jd = a.jobs[job];
+        const int64_t st = a.job_stats[(int64_t)jd.order * JS_COUNT + JS_STATUS];
+        if (st == 2 && a.inst_status[jd.inst] == 2) next[atomicAdd(&n_active[(e + 1) & 1], 1)] = job;
+    }
+}
+
+__global__ void k_epoch_advance(int32_t* epoch, const int32_t* n_active, int32_t* remaining)
+{
+    if (threadIdx.x == 0) {
+        const int32_t e = *epoch;
+        *remaining = n_active[(e + 1) & 1];
+        *epoch = e + 1;
+    }
+}
+
+hipError_t launch_epoch_begin(const SolveArgs& a, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_epoch_begin, dim3(1), dim3(64), 0, st, a.queue_head, a.epoch, (int32_t*)a.n_active);
+    return hipGetLastError();
+}
+
+hipError_t launch_epoch_end(const SolveArgs& a, int32_t* remaining, hipStream_t st)
+{
+    hipLaunchKernelGGL(k_epoch_end, dim3(std::max(1, std::min(1024, (a.n_jobs + 255) / 256))), dim3(256), 0, st, a,
+                       (int32_t*)a.active, (int32_t*)a.n_active);
+    hipLaunchKernelGGL(k_epoch_advance, dim3(1), dim3(64), 0, st, (int32_t*)a.epoch, a.n_active, remaining);
+    return hipGetLastError();
 }
 
 size_t solve_state_lds_bytes(int32_t max_n) { return Layout::state_bytes(max_n); }

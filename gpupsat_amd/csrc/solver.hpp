@@ -43,7 +43,8 @@ struct SolveArgs {
     const int64_t* var_base;        // [n_inst] first var slot of each instance in per-var arrays
     const float* act_init;          // [sum n] initial VSIDS scores (null: zero)
     const int8_t* phase_init;       // [sum n] initial phases 0/1 (null: opts.phase_init)
-    int32_t* queue_head;            // device work-queue counter
+    int32_t* queue_head;            // device work-queue counter (counts chunks)
+    int32_t chunk;                  // consecutive jobs handed out per queue grab
     int32_t* inst_status;           // [n_inst] 2 undecided, 0 SAT, 1 UNSAT
     int32_t* inst_refuted;          // [n_inst] refuted cubes
     int32_t* inst_undef;            // [n_inst] cubes stopped by a budget
@@ -51,6 +52,13 @@ struct SolveArgs {
     uint32_t* model_rows;           // [total_words] model of each SAT instance (literal bitmap)
     int64_t* job_stats;             // [n_jobs * JS_COUNT]
     int32_t* var_bumps;             // [sum n] conflict-analysis participation per var (null: off)
+    // restart epochs: the solve is a graph replayed once per epoch (all null otherwise)
+    const int32_t* epoch;           // device epoch counter
+    const int32_t* active;          // [2 * n_jobs] positions of this epoch's jobs, by parity
+    const int32_t* n_active;        // [2] their numbers
+    const int64_t* epoch_budget;    // conflict budget of every epoch
+    float* act_store;               // [sum n] activities and phases kept by stopped jobs
+    int8_t* phase_store;
     int32_t* ws;                    // global workspace
     int64_t ws_ints_per_wave;
     // per-wave clause-part layout (ints), identical in LDS and global mode
@@ -60,6 +68,8 @@ struct SolveArgs {
 };
 
 size_t solve_state_lds_bytes(int32_t max_n);
+hipError_t launch_epoch_begin(const SolveArgs& a, hipStream_t st);
+hipError_t launch_epoch_end(const SolveArgs& a, int32_t* remaining, hipStream_t st);
 hipError_t launch_solve(const SolveArgs& a, bool lds_clauses, int waves, size_t lds_bytes, hipStream_t st);
 
 }  // namespace gpsat

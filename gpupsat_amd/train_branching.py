@@ -69,17 +69,21 @@ def compare(batch, theta, opts: SolveOpts, cube_depth: int = 0):
     infer_s = time.perf_counter() - t0
     model.close()
     if cube_depth:
-        plain_vars = [cube_vars(batch.instance(i), strategy=1) [:cube_depth] for i in range(N)]
+        plain_vars = [cube_vars(batch.instance(i), strategy=1)[:cube_depth] for i in range(N)]
         learned_vars = Lr.learned_cubes(per, cube_depth)
         jp = cube_jobs(batch, plain_vars)
         jl = cube_jobs(batch, learned_vars)
         rp = gb.solve(opts, jobs=jp, want_models=False)
+        log(f"  plain cubes: {rp['info']['kernel_ms']:.0f} ms")
         rl = gb.solve(opts, jobs=jl, act_init=act, phase_init=phase, want_models=False)
+        log(f"  learned cubes: {rl['info']['kernel_ms']:.0f} ms")
         cp, tp = _per_instance(rp, jp[0], N)
         cl, tl = _per_instance(rl, jl[0], N)
     else:
         rp = gb.solve(opts, want_models=False)
+        log(f"  plain: {rp['info']['kernel_ms']:.0f} ms")
         rl = gb.solve(opts, act_init=act, phase_init=phase, want_models=False)
+        log(f"  learned: {rl['info']['kernel_ms']:.0f} ms")
         ar = np.arange(N)
         cp, tp = _per_instance(rp, ar, N)
         cl, tl = _per_instance(rl, ar, N)
@@ -107,12 +111,19 @@ def evaluate(theta, n_random: int, n_color: int, budget_conflicts: int, budget_m
     opts = SolveOpts.default(max_conflicts=budget_conflicts, max_ms=budget_ms)
     rnd = W.random_3sat_batch(n_random, 300, 1278, seed=777)
     col = W.coloring_batch(n_color, seed=778)
-    res = {
-        "random_3sat_n300": compare(rnd, theta, opts),
-        "coloring_er150": compare(col, theta, opts),
-    }
+    res = {}
+    log(f"eval: {n_random} random 3-SAT n=300 m=1278")
+    res["random_3sat_n300"] = compare(rnd, theta, opts)
+    log(json.dumps(res["random_3sat_n300"]))
+    log(f"eval: {n_color} colourings")
+    res["coloring_er150"] = compare(col, theta, opts)
+    log(json.dumps(res["coloring_er150"]))
     if cube_depth:
-        res["random_3sat_n300_cubes"] = compare(rnd, theta, opts, cube_depth=cube_depth)
+        # every cube job gets a quarter of the single-job time budget
+        copts = SolveOpts.default(max_conflicts=budget_conflicts, max_ms=budget_ms / 4)
+        log(f"eval: cubes of depth {cube_depth}")
+        res["random_3sat_n300_cubes"] = compare(rnd, theta, copts, cube_depth=cube_depth)
+        log(json.dumps(res["random_3sat_n300_cubes"]))
     return res
 
 

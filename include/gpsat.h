@@ -154,6 +154,14 @@ typedef struct gpsat_solve_opts {
     int32_t waves;           /* persistent wavefronts (0: automatic) */
     int32_t clause_mem;      /* clause part of the job state: 0 automatic, 1 LDS, 2 HBM */
     int32_t verify;          /* 1: re-check every SAT model with the clause-evaluation kernel */
+    int32_t chunk;           /* jobs per queue grab (0: automatic); cubes of one instance in one
+                                chunk reuse its watch lists on the wavefront */
+    int64_t epoch_conflicts; /* > 0: restart epochs.  The solve becomes a HIP graph (queue reset,
+                                persistent solve, re-queue of stopped jobs) replayed once per
+                                epoch; in epoch e a job may use epoch_conflicts * growth^e
+                                conflicts, and a stopped job keeps its activities and phases */
+    float epoch_growth;      /* budget growth per epoch (default 2) */
+    int32_t max_epochs;      /* epochs before the remaining jobs are UNDEF (default 16) */
 } gpsat_solve_opts;
 
 typedef struct gpsat_solve_info {
@@ -164,6 +172,8 @@ typedef struct gpsat_solve_info {
     int64_t lds_bytes;       /* per wavefront */
     int64_t workspace_bytes;
     int32_t n_sat, n_unsat, n_undef, n_verified, n_errors;
+    int32_t epochs;          /* graph replays in epoch mode (1 otherwise) */
+    int32_t chunk;           /* jobs per queue grab that was used */
 } gpsat_solve_info;
 
 int gpsat_solve_opts_default(gpsat_solve_opts *o);

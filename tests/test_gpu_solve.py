@@ -95,11 +95,13 @@ def test_pigeonhole_and_coloring(orc):
     assert st[5:] == [truth(orc, f) for f in fs[5:]]
 
 
-def test_cubes_cover_the_search_space(orc):
+@pytest.mark.parametrize("chunk", [0, 5, 16])
+def test_cubes_cover_the_search_space(orc, chunk):
     """Every instance split into 2^d cubes: SAT iff some cube is SAT, UNSAT only when
     every cube is refuted (the reference's parallel path can stop early,
-    src/SATSolver/Parallelizer.cu:60-74)."""
-    print("[solve] cube jobs", flush=True)
+    src/SATSolver/Parallelizer.cu:60-74).  chunk > 1 makes a wavefront run several
+    cubes of one instance in a row, reusing its watch lists."""
+    print(f"[solve] cube jobs chunk={chunk}", flush=True)
     b = W.random_3sat_batch(30, 50, 213, seed=11)
     ji, co, cl = [], [0], []
     for i in range(b.n_inst):
@@ -110,7 +112,7 @@ def test_cubes_cover_the_search_space(orc):
             cl.extend(cube.tolist())
             co.append(len(cl))
     with G.GpuBatch(b) as gb:
-        res = gb.solve(opts(), jobs=(ji, co, cl))
+        res = gb.solve(opts(chunk=chunk), jobs=(ji, co, cl))
         check_models(orc, gb, res, b)
     assert res["status"].tolist() == [truth(orc, b.instance(i)) for i in range(b.n_inst)]
     js = res["job_stats"]
@@ -120,6 +122,26 @@ def test_cubes_cover_the_search_space(orc):
             assert (mine == G.UNSAT).all()
         else:
             assert (mine == G.SAT).any()
+
+
+def test_cube_reuse_with_unit_clauses(orc):
+    """Formulas with unit clauses, split into cubes and run 8 cubes per wavefront: the
+    units stay at the base of the trail across cubes."""
+    print("[solve] cube reuse with units", flush=True)
+    from gpupsat_amd.formula import Formula
+    rng = np.random.default_rng(3)
+    fs = []
+    for i in range(12):
+        f = W.random_3sat_batch(1, 40, 150 + 5 * i, seed=50 + i).instance(0)
+        units = [[int(2 * v + rng.integers(0, 2))] for v in rng.choice(40, 3, replace=False)]
+        fs.append(Formula.from_clauses(40, f.clauses() + units))
+    b = Batch.from_formulas(fs)
+    vars_ = [np.array([5, 9, 13], dtype=np.int32)] * b.n_inst
+    jobs = G.cube_jobs(b, vars_)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(chunk=8), jobs=jobs)
+        check_models(orc, gb, res, b)
+    assert res["status"].tolist() == [truth(orc, f) for f in fs]
 
 
 def test_budget_gives_undef(orc):
@@ -141,3 +163,24 @@ def test_mixed_sizes_with_budget(orc):
         if res["status"][i] == G.UNSAT:
             st, _, _ = orc.solve_cdcl(b.instance(i), max_conflicts=2_000_000)
             assert st == G.UNSAT
+
+
+@pytest.mark.parametrize("cubes", [False, True])
+def test_restart_epochs_graph(orc, cubes):
+    """Restart epochs: the solve is a HIP graph replayed per epoch with a doubling
+    conflict budget; verdicts equal the oracle's and the single-launch solve's."""
+    print(f"[solve] restart epochs cubes={cubes}", flush=True)
+    fs = [W.random_3sat_batch(1, 90, 383, seed=200 + i).instance(0) for i in range(40)]
+    fs.append(W.pigeonhole_formula(7, 6))
+    b = Batch.from_formulas(fs)
+    jobs = None
+    if cubes:
+        jobs = G.cube_jobs(b, [np.array([0, 1], dtype=np.int32)] * b.n_inst)
+    with G.GpuBatch(b) as gb:
+        eager = gb.solve(opts(), jobs=jobs)
+        res = gb.solve(opts(epoch_conflicts=40, epoch_growth=2.0, max_epochs=20), jobs=jobs)
+        check_models(orc, gb, res, b)
+    assert res["info"]["epochs"] > 1
+    exp = [truth(orc, f) for f in fs]
+    assert res["status"].tolist() == exp
+    assert eager["status"].tolist() == exp
