@@ -60,6 +60,34 @@ def synchronize():
     _check(lib().gpsat_device_synchronize())
 
 
+def tune_get(key: str) -> int:
+    v = ctypes.c_int32()
+    _check(lib().gpsat_tune_get(key.encode(), ctypes.byref(v)))
+    return v.value
+
+
+def tune_set(key: str, value: int):
+    _check(lib().gpsat_tune_set(key.encode(), ctypes.c_int32(int(value))))
+
+
+class tuned:
+    """Context manager: set launch-shape keys (gpsat_tune_set) and restore them on exit."""
+
+    def __init__(self, **kw):
+        self.kw = kw
+        self.old = {}
+
+    def __enter__(self):
+        for k, v in self.kw.items():
+            self.old[k] = tune_get(k)
+            tune_set(k, v)
+        return self
+
+    def __exit__(self, *a):
+        for k, v in self.old.items():
+            tune_set(k, v)
+
+
 class DevBuf:
     """A device allocation owned by Python (freed on close / garbage collection)."""
 

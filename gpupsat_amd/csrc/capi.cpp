@@ -19,6 +19,8 @@ Tuning& tuning()
     static Tuning t = [] {
         Tuning x;
         if (const char* e = getenv("GPSAT_TUNE_EVAL_STAGE")) x.eval_stage_arena = atoi(e);
+        if (const char* e = getenv("GPSAT_TUNE_EVAL_FAST3")) x.eval_fast3 = atoi(e);
+        if (const char* e = getenv("GPSAT_TUNE_EVAL_WPB")) x.eval_waves_per_block = atoi(e);
         return x;
     }();
     return t;
@@ -48,6 +50,31 @@ extern "C" {
 
 const char* gpsat_last_error(void) { return g_last_error.c_str(); }
 const char* gpsat_version(void) { return "v0.0.1"; }
+
+static int* tune_slot(const char* key)
+{
+    if (!key) return nullptr;
+    if (!strcmp(key, "eval_fast3")) return &tuning().eval_fast3;
+    if (!strcmp(key, "eval_stage_arena")) return &tuning().eval_stage_arena;
+    if (!strcmp(key, "eval_waves_per_block")) return &tuning().eval_waves_per_block;
+    return nullptr;
+}
+
+int gpsat_tune_set(const char* key, int32_t value)
+{
+    int* p = tune_slot(key);
+    ARG_CHECK(p, "unknown tuning key");
+    *p = value;
+    return GPSAT_OK;
+}
+
+int gpsat_tune_get(const char* key, int32_t* value)
+{
+    int* p = tune_slot(key);
+    ARG_CHECK(p && value, "unknown tuning key or null output");
+    *value = *p;
+    return GPSAT_OK;
+}
 
 int gpsat_device_count(int32_t* count)
 {
@@ -167,6 +194,7 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
     d.h_coff.resize(M + n_inst);
     d.h_lits.assign(lits ? lits + l0 : nullptr, lits ? lits + l0 + L : nullptr);
     int64_t wbase = 0;
+    bool fast3 = true;
     for (int32_t i = 0; i < n_inst; i++) {
         const int64_t cb = inst_clause_off[i], ce = inst_clause_off[i + 1];
         if (ce < cb || n_vars[i] < 0) {
@@ -197,11 +225,15 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
                 return GPSAT_ERR_ARG;
             }
         }
+        bool all3 = true;
+        for (int64_t c = cb; c < ce && all3; c++) all3 = clause_off[c + 1] - clause_off[c] == 3;
+        fast3 = fast3 && all3 && fast3_fits(mi.n, mi.m);
         wbase += words_per_row(mi.n);
         d.max_n = std::max(d.max_n, mi.n);
         d.max_m = std::max(d.max_m, mi.m);
     }
     d.total_words = wbase;
+    d.fast3 = fast3 && n_inst > 0;
     hipError_t e = hipGetDevice(&d.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
     if (e == hipSuccess && n_inst) e = hipMalloc(&d.meta, sizeof(InstMeta) * n_inst);

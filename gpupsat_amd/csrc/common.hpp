@@ -25,48 +25,97 @@ __host__ __device__ inline int32_t words_per_row(int32_t n) { return (2 * n + 31
 
 __device__ __forceinline__ int lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
-// xor-swizzle inside each 32-lane group (ds_swizzle bitmask mode: and 0x1f, xor S).
-template <int S>
-__device__ __forceinline__ uint32_t swz_xor(uint32_t x)
+// Bit-field insert: (mask & a) | (~mask & b) in one v_bfi_b32 (the compiler otherwise
+// splits the select across neighbouring transpose stages).
+__device__ __forceinline__ uint32_t bfi(uint32_t mask, uint32_t a, uint32_t b)
 {
-    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (S << 10));
+    uint32_t r;
+    asm("v_bfi_b32 %0, %1, %2, %3" : "=v"(r) : "v"(mask), "v"(a), "v"(b));
+    return r;
 }
 
 // 32x32 bit-matrix transpose held one row per lane in each 32-lane half: stage S swaps
-// the off-diagonal SxS blocks of every 2Sx2S block.  Per stage a lane takes its partner's
-// row (lane ^ S), rotates it by S one way or the other (a per-lane rotate amount) and
-// keeps its own bits outside the half it receives.
+// the off-diagonal SxS blocks of every 2Sx2S block, between lanes i and i ^ S.  The
+// partner's row arrives through the VALU's cross-lane paths, not through LDS:
+//   S = 16  v_permlane16_swap (rows of 16 lanes trade places);
+//   S = 8   DPP row_ror:8 (lane i reads lane (i - 8) mod 16 = i ^ 8);
+//   S = 4   two DPP row_ror moves under bank masks (banks 0, 2 read i + 4, banks 1, 3 i - 4);
+//   S = 2, 1  DPP quad_perm [2,3,0,1] and [1,0,3,2].
+// For S = 16 and 8 the blocks are whole bytes and one v_perm_b32 with a per-lane byte
+// selector merges own and partner bytes; for S = 4, 2, 1 the partner's row is rotated by
+// S one way or the other (a per-lane amount) and merged with a bit-field insert.
 struct Transposer {
-    uint32_t rot[5];
-    uint32_t keep[5];
+    uint32_t sel16, sel8;
+    uint32_t rot[3];
+    uint32_t keep[3];
     __device__ explicit Transposer(int lane)
     {
-        const uint32_t H[5] = {0xFFFF0000u, 0xFF00FF00u, 0xF0F0F0F0u, 0xCCCCCCCCu, 0xAAAAAAAAu};
+        // v_perm_b32 byte selectors over {src0 (bytes 4-7), src1 (bytes 0-3)}
+        sel16 = (lane & 16) ? 0x07060302u : 0x05040100u;   // [y2 y3 x2 x3] | [x0 x1 y0 y1]
+        sel8 = (lane & 8) ? 0x03070105u : 0x06020400u;     // [y1 x1 y3 x3] | [x0 y0 x2 y2]
+        const uint32_t H[3] = {0xF0F0F0F0u, 0xCCCCCCCCu, 0xAAAAAAAAu};
 #pragma unroll
-        for (int k = 0; k < 5; k++) {
-            const uint32_t S = 16u >> k;
+        for (int k = 0; k < 3; k++) {
+            const uint32_t S = 4u >> k;
             const bool upper = (lane & S) != 0;
             rot[k] = upper ? S : 32u - S;       // rotate right by S, or left by S
             keep[k] = upper ? H[k] : ~H[k];
         }
     }
     template <int K>
-    __device__ __forceinline__ uint32_t stage(uint32_t x) const
+    __device__ __forceinline__ uint32_t bit_stage(uint32_t x, uint32_t y) const
     {
-        const uint32_t y = swz_xor<(16 >> K)>(x);
         const uint32_t t = __builtin_amdgcn_alignbit(y, y, rot[K]);
-        return (x & keep[K]) | (t & ~keep[K]);
+        return bfi(keep[K], x, t);
     }
     // afterwards lane r holds column r (bit c = old row c, bit r)
     __device__ __forceinline__ uint32_t operator()(uint32_t x) const
     {
-        x = stage<0>(x);
-        x = stage<1>(x);
-        x = stage<2>(x);
-        x = stage<3>(x);
-        return stage<4>(x);
+        // r[0]: rows 0 and 2 keep x, rows 1 and 3 receive the row below; r[1]: the reverse,
+        // so {r[1], r[0]} holds own and partner bytes in a lane-dependent order
+        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+        x = __builtin_amdgcn_perm(r[1], r[0], sel16);
+        uint32_t y = __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);      // row_ror:8
+        x = __builtin_amdgcn_perm(y, x, sel8);
+        y = __builtin_amdgcn_update_dpp(0u, x, 0x12C, 0xF, 0x5, false);         // row_ror:12, banks 0, 2
+        y = __builtin_amdgcn_update_dpp(y, x, 0x124, 0xF, 0xA, false);          // row_ror:4, banks 1, 3
+        x = bit_stage<0>(x, y);
+        y = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);                 // quad_perm [2,3,0,1]
+        x = bit_stage<1>(x, y);
+        y = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);                 // quad_perm [1,0,3,2]
+        return bit_stage<2>(x, y);
     }
 };
+
+// Orders this wavefront's LDS accesses: LDS requests of one wavefront are served in
+// issue order, so it only has to keep the compiler from moving LDS accesses across it.
+// Kernels whose workgroups hold several independent wavefronts (each with a private LDS
+// region) use it instead of a workgroup barrier.
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// OR of x over the 64 lanes, returned in every lane, through DPP and the permlane swaps
+// (no LDS): quads, then rows of 16, then the two rows of each half, then the halves.
+__device__ __forceinline__ uint32_t wave_or32(uint32_t x)
+{
+    x |= __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
+    x |= __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
+    x |= __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false);   // row_ror:4
+    x |= __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);   // row_ror:8
+    const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
+    x = r[0] | r[1];
+    const auto q = __builtin_amdgcn_permlane32_swap(x, x, false, false);
+    return q[0] | q[1];
+}
+
+__device__ __forceinline__ uint64_t wave_or64(uint64_t x)
+{
+    return (uint64_t)wave_or32((uint32_t)x) | ((uint64_t)wave_or32((uint32_t)(x >> 32)) << 32);
+}
 
 __device__ __forceinline__ uint32_t shfl_xor32(uint32_t x)
 {

@@ -36,14 +36,14 @@ __device__ __forceinline__ void build_literal_masks(const uint32_t* __restrict__
                                                     uint32_t* stage, uint32_t* lt32, int lane, const Transposer& tp)
 {
     bulk_to_lds(stage, rows, A * W, lane);
-    __syncthreads();
+    wave_sync();
     const int half = lane >> 5, i = lane & 31;
     for (int j = 0; j < W; j++) {
         uint32_t x = (lane < A) ? stage[lane * W + j] : 0u;
         x = tp(x);
         lt32[2 * (32 * j + i) + half] = x;
     }
-    __syncthreads();
+    wave_sync();
 }
 
 struct ClauseMasks {
@@ -103,38 +103,30 @@ __device__ __forceinline__ void count_group(uint64_t b0, uint64_t b1, const Tran
     }
 }
 
-__device__ __forceinline__ uint64_t lds_mask(const uint32_t* w) { return (uint64_t)w[0] | ((uint64_t)w[1] << 32); }
-
-__device__ __forceinline__ void publish(uint32_t* conf, uint64_t x)
-{
-    if (x) {
-        atomicOr(&conf[0], (uint32_t)x);
-        atomicOr(&conf[1], (uint32_t)(x >> 32));
-    }
-}
-
 // ---------------------------------------------------------------------------------------
 // DO_EVAL: write #SAT/#UNSAT/#UNIT per (instance, assignment) for the input rows.
 // DO_BCP:  write the closure rows, conflict flags and implied-literal counts.
 // STAGE: the instance's clause arena (literals and offsets) is copied to LDS first, so
 // every clause group reads LDS only.
 template <bool DO_EVAL, bool DO_BCP, bool STAGE>
-__global__ __launch_bounds__(64) void k_eval_bcp(const InstMeta* __restrict__ meta, const int32_t* __restrict__ lits,
-                                                  const int32_t* __restrict__ coff, const uint32_t* __restrict__ rows,
-                                                  int A, int32_t* __restrict__ counts, uint32_t* __restrict__ out_rows,
-                                                  int32_t* __restrict__ conflict, int32_t* __restrict__ props,
-                                                  int32_t* __restrict__ passes_out)
+__global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ meta, const int32_t* __restrict__ lits,
+                                                   const int32_t* __restrict__ coff, const uint32_t* __restrict__ rows,
+                                                   int A, int32_t* __restrict__ counts, uint32_t* __restrict__ out_rows,
+                                                   int32_t* __restrict__ conflict, int32_t* __restrict__ props,
+                                                   int32_t* __restrict__ passes_out, int n_inst, int lds_words)
 {
-    extern __shared__ __align__(16) uint32_t smem[];
-    const int inst = blockIdx.x;
-    const int lane = threadIdx.x;
+    extern __shared__ __align__(16) uint32_t lds_all[];
+    const int inst = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    const int lane = threadIdx.x % kWave;
+    if (inst >= n_inst) return;
+    uint32_t* smem = lds_all + (size_t)(threadIdx.x / kWave) * lds_words;
     const Transposer tp(lane);
     const InstMeta mi = meta[inst];
     const int W = words_per_row(mi.n);
     const int m = mi.m;
     uint32_t* lt32 = smem;                       // 64*W words (2 per literal slot)
     uint32_t* stage = smem + 64 * W;             // A*W words
-    uint32_t* conf = stage + 64 * W;             // 2 words
+    uint32_t* conf = stage + 64 * W;             // 4 spare words before the staged arena
     const uint4* lt4 = reinterpret_cast<const uint4*>(lt32);
     uint64_t* lt64 = reinterpret_cast<uint64_t*>(lt32);
     const int64_t row0 = (int64_t)A * mi.word_base;
@@ -151,8 +143,7 @@ __global__ __launch_bounds__(64) void k_eval_bcp(const InstMeta* __restrict__ me
     }
 
     build_literal_masks(rows + row0, A, W, stage, lt32, lane, tp);
-    if (lane < 2) conf[lane] = 0u;
-    __syncthreads();
+    wave_sync();
 
     const int ngroups = (m + kWave - 1) / kWave;
     if (DO_EVAL) {
@@ -189,16 +180,15 @@ __global__ __launch_bounds__(64) void k_eval_bcp(const InstMeta* __restrict__ me
     if (DO_BCP) {
         const uint64_t amask = A >= 64 ? ~0ull : ((1ull << A) - 1ull);
         // contradictory inputs (both polarities true) are conflicts from the start
+        uint64_t frozen;
         {
             uint64_t cm = 0;
             for (int v = lane; v < 16 * W; v += kWave) {
                 const uint4 q = lt4[v];
                 cm |= ((uint64_t)q.x | ((uint64_t)q.y << 32)) & ((uint64_t)q.z | ((uint64_t)q.w << 32));
             }
-            publish(conf, cm);
-            __syncthreads();
+            frozen = wave_or64(cm);
         }
-        uint64_t frozen = lds_mask(conf);
         int pass = 0;
         bool changed = (frozen & amask) != amask;
         while (changed) {
@@ -242,26 +232,20 @@ __global__ __launch_bounds__(64) void k_eval_bcp(const InstMeta* __restrict__ me
                 }
                 // a conflicting assignment stops propagating as soon as it is found
                 if (ballot(newconf != 0)) {
-                    publish(conf, newconf);
-                    __syncthreads();
-                    frozen = lds_mask(conf);
+                    frozen |= wave_or64(newconf);
                     if ((frozen & amask) == amask) break;
                 }
             }
             changed = ballot(my_change) != 0ull && (frozen & amask) != amask;
-            __syncthreads();
+            wave_sync();
         }
         // complementary implications are conflicts too
-        {
-            uint64_t cm = 0;
-            for (int v = lane; v < 16 * W; v += kWave) {
-                const uint4 q = lt4[v];
-                cm |= ((uint64_t)q.x | ((uint64_t)q.y << 32)) & ((uint64_t)q.z | ((uint64_t)q.w << 32));
-            }
-            publish(conf, cm);
-            __syncthreads();
+        uint64_t cm = 0;
+        for (int v = lane; v < 16 * W; v += kWave) {
+            const uint4 q = lt4[v];
+            cm |= ((uint64_t)q.x | ((uint64_t)q.y << 32)) & ((uint64_t)q.z | ((uint64_t)q.w << 32));
         }
-        const uint64_t cmask = lds_mask(conf);
+        const uint64_t cmask = frozen | wave_or64(cm);
         const bool my_conflict = (cmask >> lane) & 1ull;
         const int half = lane >> 5, i = lane & 31;
         uint32_t np = 0;
@@ -285,29 +269,280 @@ __global__ __launch_bounds__(64) void k_eval_bcp(const InstMeta* __restrict__ me
     }
 }
 
-size_t eval_bcp_lds_bytes(int max_n) { return (size_t)(128 * words_per_row(max_n) + 4) * 4 + 16; }
+// ---------------------------------------------------------------------------------------
+// Register-resident path for instances whose clauses all hold exactly three literals,
+// with n <= 256 (W <= 16 words per row) and m <= 1024 (<= 16 clause groups).  Every
+// global load of the instance (its A rows and its 3m literals) is issued before the
+// first one is used; each lane then keeps its 16 clauses as packed 10-bit literal
+// triples in VGPRs and its input row in VGPRs, so the BCP passes touch LDS only.
+//
+// LDS holds the literal masks twice: slot l (16 bytes) = {T(l), T(l ^ 1)}, the true and
+// the false mask of literal l, so a clause literal costs one 16-byte read and no select.
+// A propagated literal l is OR-ed into T(l) of slot l and into the false half of slot
+// l ^ 1.  The first 64*W words double as the row staging area.  LDS: (128*W + 4) words.
+constexpr int kFastW = kFast3MaxWords;
+constexpr int kFastG = kFast3MaxGroups;
 
-static size_t stage_lds_bytes(const DeviceBatch& b)
+__device__ __forceinline__ uint64_t u64(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
+
+template <bool DO_EVAL, bool DO_BCP>
+__global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ meta, const int32_t* __restrict__ lits,
+                                                    const uint32_t* __restrict__ rows, int A, int32_t* __restrict__ counts,
+                                                    uint32_t* __restrict__ out_rows, int32_t* __restrict__ conflict,
+                                                    int32_t* __restrict__ props, int32_t* __restrict__ passes_out,
+                                                    int n_inst, int lds_words)
+{
+    extern __shared__ __align__(16) uint32_t lds_all[];
+    const int inst = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    const int lane = threadIdx.x % kWave;
+    if (inst >= n_inst) return;
+    uint32_t* smem = lds_all + (size_t)(threadIdx.x / kWave) * lds_words;
+    const InstMeta mi = meta[inst];
+    const int W = words_per_row(mi.n);
+    const int m = mi.m;
+    const int G = (m + kWave - 1) / kWave;
+    uint32_t* dup = smem;                        // 32*W slots of 4 words
+    const uint4* dup4 = reinterpret_cast<const uint4*>(dup);
+    uint64_t* dup64 = reinterpret_cast<uint64_t*>(dup);
+    const int64_t row0 = (int64_t)A * mi.word_base;
+    const int32_t* ilits = lits + mi.lit_base;
+
+    // rows: coalesced copy of the A*W words into LDS; literals: straight into VGPRs
+    const uint32_t* src = rows + row0;
+    const int nw = A * W;
+    uint32_t rbuf[kFastW];
+#pragma unroll
+    for (int u = 0; u < kFastW; u++) {
+        const int w = u * kWave + lane;
+        rbuf[u] = (w < nw) ? __builtin_nontemporal_load(src + w) : 0u;
+    }
+    uint32_t pk[kFastG];
+#pragma unroll
+    for (int g = 0; g < kFastG; g++) {
+        const int c = g * kWave + lane;
+        pk[g] = 0u;
+        if (c < m) {
+            const int32_t* p = ilits + 3 * c;
+            const uint32_t a = (uint32_t)__builtin_nontemporal_load(p);
+            const uint32_t b = (uint32_t)__builtin_nontemporal_load(p + 1);
+            const uint32_t d = (uint32_t)__builtin_nontemporal_load(p + 2);
+            pk[g] = a | (b << 10) | (d << 20);
+        }
+    }
+#pragma unroll
+    for (int u = 0; u < kFastW; u++) {
+        const int w = u * kWave + lane;
+        if (w < nw) dup[w] = rbuf[u];
+    }
+    wave_sync();
+    uint32_t in[kFastW];
+#pragma unroll
+    for (int j = 0; j < kFastW; j++) in[j] = (j < W && lane < A) ? dup[lane * W + j] : 0u;
+    wave_sync();
+    const Transposer tp(lane);
+    const int half = lane >> 5, i = lane & 31;
+#pragma unroll
+    for (int j = 0; j < kFastW; j++) {
+        if (j < W) {
+            const uint32_t x = tp(in[j]);
+            const int l = 32 * j + i;
+            dup[4 * l + half] = x;
+            dup[4 * (l ^ 1) + 2 + half] = x;
+        }
+    }
+    wave_sync();
+
+    uint64_t unsat_any = 0;      // rows with a clause falsified by the input
+    if (DO_EVAL) {
+        uint32_t acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        // software pipeline: the three 16-byte reads of group g+1 are issued before group
+        // g is evaluated; lanes past m read slot 0 (pk = 0) and are masked afterwards
+        uint4 na = dup4[pk[0] & 1023u], nb = dup4[(pk[0] >> 10) & 1023u], nc = dup4[pk[0] >> 20];
+#pragma unroll
+        for (int g = 0; g < kFastG; g++) {
+            if (g < G) {
+                const uint4 qa = na, qb = nb, qc = nc;
+                if (g + 1 < kFastG) {
+                    const uint32_t p = pk[g + 1];
+                    na = dup4[p & 1023u];
+                    nb = dup4[(p >> 10) & 1023u];
+                    nc = dup4[p >> 20];
+                }
+                const uint64_t valid = (g * kWave + lane < m) ? ~0ull : 0ull;
+                const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
+                const uint64_t T = (u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y)) & valid;
+                const uint64_t all = Fa & Fb & Fc & valid;
+                const uint64_t unit = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T & valid;
+                const uint64_t unsat = all & ~T;
+                unsat_any |= unsat;
+                count_group(T | unit, unsat | unit, tp, acc);
+            }
+        }
+        uint32_t tot[3];
+#pragma unroll
+        for (int s = 0; s < 3; s++) {
+            const uint32_t lo = acc[0][s] + shfl_xor32(acc[0][s]);
+            const uint32_t hi = acc[1][s] + shfl_xor32(acc[1][s]);
+            tot[s] = (lane < 32) ? lo : hi;
+        }
+        if (lane < A) {
+            int32_t* o = counts + ((int64_t)inst * A + lane) * 3;
+            o[0] = (int32_t)tot[0];
+            o[1] = (int32_t)tot[1];
+            o[2] = (int32_t)tot[2];
+        }
+    }
+
+    if (DO_BCP) {
+        const uint64_t amask = A >= 64 ? ~0ull : ((1ull << A) - 1ull);
+        // contradictory inputs, and inputs that already falsify a clause, are conflicts
+        uint64_t frozen0;
+        {
+            uint64_t cm = unsat_any;
+            for (int v = lane; v < 16 * W; v += kWave) {
+                const uint4 q = dup4[2 * v];
+                cm |= u64(q.x, q.y) & u64(q.z, q.w);
+            }
+            frozen0 = wave_or64(cm);
+        }
+        uint64_t frozen = frozen0;
+        int pass = 0;
+        bool changed = (frozen & amask) != amask;
+        while (changed) {
+            pass++;
+            bool my_change = false;
+            // the reads of group g+1 are issued before group g applies its implications,
+            // so they may miss them until the next pass (the fixpoint is unchanged: the
+            // loop ends only after a pass without a single write, or with every row in
+            // conflict).  The empty asm keeps the unpacked literals inside the pass
+            // instead of hoisted into 48 live registers.
+            uint32_t pn = pk[0];
+            asm volatile("" : "+v"(pn));
+            uint4 na = dup4[pn & 1023u], nb = dup4[(pn >> 10) & 1023u], nc = dup4[pn >> 20];
+#pragma unroll
+            for (int g = 0; g < kFastG; g++) {
+                if (g < G) {
+                    const uint32_t p = pn;
+                    const uint4 qa = na, qb = nb, qc = nc;
+                    if (g + 1 < kFastG) {
+                        pn = pk[g + 1];
+                        asm volatile("" : "+v"(pn));
+                        na = dup4[pn & 1023u];
+                        nb = dup4[(pn >> 10) & 1023u];
+                        nc = dup4[pn >> 20];
+                    }
+                    const uint64_t valid = (g * kWave + lane < m) ? ~frozen : 0ull;
+                    const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
+                    const uint64_t T = u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y);
+                    const uint64_t all = Fa & Fb & Fc;
+                    const uint64_t newconf = all & ~T & valid;
+                    const uint64_t U = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T & valid;
+                    if (U) {
+                        const uint32_t la = p & 1023u, lb = (p >> 10) & 1023u, lc = p >> 20;
+                        const uint64_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
+                        if (Ua) {
+                            atomicOr((unsigned long long*)&dup64[2 * la], (unsigned long long)Ua);
+                            atomicOr((unsigned long long*)&dup64[2 * (la ^ 1u) + 1], (unsigned long long)Ua);
+                        }
+                        if (Ub) {
+                            atomicOr((unsigned long long*)&dup64[2 * lb], (unsigned long long)Ub);
+                            atomicOr((unsigned long long*)&dup64[2 * (lb ^ 1u) + 1], (unsigned long long)Ub);
+                        }
+                        if (Uc) {
+                            atomicOr((unsigned long long*)&dup64[2 * lc], (unsigned long long)Uc);
+                            atomicOr((unsigned long long*)&dup64[2 * (lc ^ 1u) + 1], (unsigned long long)Uc);
+                        }
+                        my_change = true;
+                    }
+                    if (ballot(newconf != 0)) {
+                        frozen |= wave_or64(newconf);
+                        if ((frozen & amask) == amask) break;
+                    }
+                }
+            }
+            changed = ballot(my_change) != 0ull && (frozen & amask) != amask;
+            wave_sync();
+        }
+        // complementary implications are conflicts too
+        uint64_t cm = 0;
+        for (int v = lane; v < 16 * W; v += kWave) {
+            const uint4 q = dup4[2 * v];
+            cm |= u64(q.x, q.y) & u64(q.z, q.w);
+        }
+        const uint64_t cmask = frozen | wave_or64(cm);
+        const bool my_conflict = (cmask >> lane) & 1ull;
+        const bool all_conflict = (cmask & amask) == amask;
+        uint32_t np = 0;
+        uint32_t out[kFastW];
+#pragma unroll
+        for (int j = 0; j < kFastW; j++) {
+            out[j] = in[j];
+            if (j < W && !all_conflict) {
+                const uint32_t x = tp(dup[4 * (32 * j + i) + half]);   // lane a: word j of row a
+                if (!my_conflict) out[j] = x;
+            }
+            np += __builtin_popcount(out[j] & ~in[j]);
+        }
+        // stage the closure rows in LDS and store them coalesced
+        wave_sync();
+#pragma unroll
+        for (int j = 0; j < kFastW; j++)
+            if (j < W && lane < A) dup[lane * W + j] = out[j];
+        wave_sync();
+        uint32_t* dst = out_rows + row0;
+#pragma unroll
+        for (int u = 0; u < kFastW; u++) {
+            const int w = u * kWave + lane;
+            if (w < nw) dst[w] = dup[w];
+        }
+        if (lane < A) {
+            conflict[(int64_t)inst * A + lane] = my_conflict ? 1 : 0;
+            props[(int64_t)inst * A + lane] = (int32_t)np;
+        }
+        if (passes_out && lane == 0) passes_out[inst] = pass;
+    }
+}
+
+// LDS words of one wavefront's region in the general kernel (literal masks, row staging,
+// conflict words; plus the clause arena when it is staged).
+static int general_lds_words(int max_n) { return 128 * words_per_row(max_n) + 8; }
+
+size_t eval_bcp_lds_bytes(int max_n) { return (size_t)general_lds_words(max_n) * 4; }
+
+static int stage_lds_words(const DeviceBatch& b)
 {
     size_t mx = 0;
     for (const InstMeta& mi : b.h_meta) {
         const size_t L = (size_t)b.h_coff[mi.coff_base + mi.m];
         mx = std::max(mx, L + (((size_t)mi.m + 4) & ~(size_t)3));
     }
-    return eval_bcp_lds_bytes(b.max_n) + 4 * mx;
+    return (general_lds_words(b.max_n) + (int)mx + 3) & ~3;   // keeps every region 16-byte aligned
 }
 
 template <bool E, bool B>
-static void launch_variant(const DeviceBatch& b, bool stage, size_t lds, const uint32_t* rows, int A, int32_t* counts,
-                           uint32_t* out_rows, int32_t* conflict, int32_t* props, int32_t* passes, hipStream_t st)
+static void launch_variant(const DeviceBatch& b, bool stage, int wpb, int lds_words, const uint32_t* rows, int A,
+                           int32_t* counts, uint32_t* out_rows, int32_t* conflict, int32_t* props, int32_t* passes,
+                           hipStream_t st)
 {
-    dim3 grid(b.n_inst), block(kWave);
+    dim3 grid((b.n_inst + wpb - 1) / wpb), block(kWave * wpb);
+    const size_t lds = (size_t)wpb * lds_words * 4;
     if (stage)
         hipLaunchKernelGGL((k_eval_bcp<E, B, true>), grid, block, lds, st, b.meta, b.lits, b.coff, rows, A, counts,
-                           out_rows, conflict, props, passes);
+                           out_rows, conflict, props, passes, b.n_inst, lds_words);
     else
         hipLaunchKernelGGL((k_eval_bcp<E, B, false>), grid, block, lds, st, b.meta, b.lits, b.coff, rows, A, counts,
-                           out_rows, conflict, props, passes);
+                           out_rows, conflict, props, passes, b.n_inst, lds_words);
+}
+
+// Wavefronts per workgroup: a CU holds at most 16 workgroups, so one-wave workgroups
+// would cap it at 16 wavefronts; the tuned value (1, 2 or 4) is clamped so the
+// workgroup's LDS stays within 64 KB.
+static int waves_per_block(int lds_words)
+{
+    int w = tuning().eval_waves_per_block;
+    w = (w >= 4) ? 4 : (w >= 2 ? 2 : 1);
+    while (w > 1 && (size_t)w * lds_words * 4 > 64 * 1024) w >>= 1;
+    return w;
 }
 
 hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, int32_t* counts, uint32_t* out_rows,
@@ -315,16 +550,33 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
                            hipStream_t st)
 {
     if (b.n_inst == 0) return hipSuccess;
+    if (b.fast3 && tuning().eval_fast3) {
+        const int lds_words = 128 * words_per_row(b.max_n) + 4;
+        const int wpb = waves_per_block(lds_words);
+        dim3 grid((b.n_inst + wpb - 1) / wpb), block(kWave * wpb);
+        const size_t lds = (size_t)wpb * lds_words * 4;
+        if (do_eval && do_bcp)
+            hipLaunchKernelGGL((k_eval_bcp3<true, true>), grid, block, lds, st, b.meta, b.lits, rows, A, counts, out_rows,
+                               conflict, props, passes, b.n_inst, lds_words);
+        else if (do_eval)
+            hipLaunchKernelGGL((k_eval_bcp3<true, false>), grid, block, lds, st, b.meta, b.lits, rows, A, counts,
+                               out_rows, conflict, props, passes, b.n_inst, lds_words);
+        else
+            hipLaunchKernelGGL((k_eval_bcp3<false, true>), grid, block, lds, st, b.meta, b.lits, rows, A, counts,
+                               out_rows, conflict, props, passes, b.n_inst, lds_words);
+        return hipGetLastError();
+    }
     // stage the arena in LDS when the largest instance keeps the wavefront under 40 KB
-    const size_t staged = stage_lds_bytes(b);
-    const bool stage = tuning().eval_stage_arena && staged <= 40 * 1024;
-    const size_t lds = stage ? staged : eval_bcp_lds_bytes(b.max_n);
+    const int staged = stage_lds_words(b);
+    const bool stage = tuning().eval_stage_arena && (size_t)staged * 4 <= 40 * 1024;
+    const int lds_words = stage ? staged : general_lds_words(b.max_n);
+    const int wpb = waves_per_block(lds_words);
     if (do_eval && do_bcp)
-        launch_variant<true, true>(b, stage, lds, rows, A, counts, out_rows, conflict, props, passes, st);
+        launch_variant<true, true>(b, stage, wpb, lds_words, rows, A, counts, out_rows, conflict, props, passes, st);
     else if (do_eval)
-        launch_variant<true, false>(b, stage, lds, rows, A, counts, out_rows, conflict, props, passes, st);
+        launch_variant<true, false>(b, stage, wpb, lds_words, rows, A, counts, out_rows, conflict, props, passes, st);
     else
-        launch_variant<false, true>(b, stage, lds, rows, A, counts, out_rows, conflict, props, passes, st);
+        launch_variant<false, true>(b, stage, wpb, lds_words, rows, A, counts, out_rows, conflict, props, passes, st);
     return hipGetLastError();
 }
 

@@ -18,6 +18,7 @@ struct DeviceBatch {
     int64_t n_clauses = 0;
     int64_t n_lits = 0;
     int64_t total_words = 0;      // sum of W_i (rows buffer holds A * total_words words)
+    bool fast3 = false;           // every instance fits the register-resident 3-literal kernel
     InstMeta* meta = nullptr;     // device [n_inst]
     int32_t* lits = nullptr;      // device [n_lits]
     int32_t* coff = nullptr;      // device [n_clauses + n_inst], per-instance relative offsets
@@ -34,8 +35,16 @@ void set_error(const std::string& msg);
 // Launch-shape choices, set by the autotuner (gpsat_tune_*) or the GPSAT_TUNE_* environment.
 struct Tuning {
     int eval_stage_arena = 0;     // copy each instance's clause arena to LDS before evaluating
+    int eval_fast3 = 1;           // use the register-resident kernel when the batch allows it
+    int eval_waves_per_block = 4; // independent wavefronts (instances) per workgroup: 1, 2 or 4
 };
 Tuning& tuning();
+
+// Limits of the register-resident evaluation kernel: every clause has exactly three
+// literals, W <= 16 words per row (n <= 256) and m <= 1024 (16 groups of 64 clauses).
+constexpr int kFast3MaxWords = 16;
+constexpr int kFast3MaxGroups = 16;
+inline bool fast3_fits(int n, int m) { return words_per_row(n) <= kFast3MaxWords && m <= 64 * kFast3MaxGroups; }
 
 size_t eval_bcp_lds_bytes(int max_n);
 hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, int32_t* counts, uint32_t* out_rows,

@@ -64,6 +64,17 @@ int gpsat_event_elapsed_ms(void *start, void *stop, float *ms);
  * stream): the practical-peak and counter-scaling probe; no reference counterpart. */
 int gpsat_copy_dev(gpsat_batch *b, void *dst_dev, const void *src_dev, int64_t bytes);
 
+/* ---- launch-shape tuning (process-wide; no reference counterpart).  Keys:
+ *   "eval_fast3"        1: evaluate batches whose clauses all have 3 literals, n <= 256 and
+ *                       m <= 1024 with the register-resident kernel (default); 0: general kernel
+ *   "eval_stage_arena"  1: the general kernel copies each clause arena to LDS first (default 0)
+ *   "eval_waves_per_block"  instances (wavefronts) per workgroup of the eval/BCP kernels:
+ *                       1, 2 or 4 (default 4)
+ * The GPSAT_TUNE_EVAL_FAST3 / GPSAT_TUNE_EVAL_STAGE / GPSAT_TUNE_EVAL_WPB environment
+ * variables set the initial values.  Returns GPSAT_ERR_ARG for an unknown key. */
+int gpsat_tune_set(const char *key, int32_t value);
+int gpsat_tune_get(const char *key, int32_t *value);
+
 /* ---- formula batches.
  * Replaces the reference's per-formula device arena CUDAClauseVec::alloc_and_copy_to_dev
  * (src/Utils/CUDAClauseVec.cu:85-118) with one CSR arena for many instances:

@@ -10,10 +10,27 @@ from gpupsat_amd.formula import Batch, Formula, pack_rows
 pytestmark = pytest.mark.gpu
 
 
+def fused(gb, rows, A):
+    """One eval+BCP launch on device buffers."""
+    rb = G.DevBuf.from_host(np.ascontiguousarray(rows, dtype=np.uint32))
+    nI = gb.n_inst
+    cb, ob = G.DevBuf(nI * A * 12), G.DevBuf(rows.size * 4)
+    fb, pb = G.DevBuf(nI * A * 4), G.DevBuf(nI * A * 4)
+    gb.eval_bcp_dev(rb, A, cb, ob, fb, pb)
+    gb.sync()
+    return (cb.to_host(np.int32, nI * A * 3).reshape(nI, A, 3), ob.to_host(np.uint32, rows.size),
+            fb.to_host(np.int32, nI * A).reshape(nI, A), pb.to_host(np.int32, nI * A).reshape(nI, A))
+
+
 def check_batch(orc, b, rows, A):
     with G.GpuBatch(b) as gb:
         counts = gb.clause_eval(rows, A)
         out, conflict, props = gb.bcp_fixpoint(rows, A)
+        f_counts, f_out, f_conflict, f_props = fused(gb, rows, A)
+    np.testing.assert_array_equal(f_counts, counts)
+    np.testing.assert_array_equal(f_out, out)
+    np.testing.assert_array_equal(f_conflict, conflict)
+    np.testing.assert_array_equal(f_props, props)
     exp_counts = orc.eval_batch(b, rows, A)
     np.testing.assert_array_equal(counts, exp_counts)
     e_out, e_conf, e_props = orc.bcp_batch(b, rows, A)
@@ -23,14 +40,56 @@ def check_batch(orc, b, rows, A):
     return counts, conflict, props
 
 
+@pytest.mark.parametrize("fast3", [1, 0])
 @pytest.mark.parametrize("p", [0.02, 0.05, 0.1, 0.3])
-def test_config2_shape_subset(orc, p):
-    print(f"[eval/bcp] n=200 m=852 N=96 A=64 p={p}", flush=True)
+def test_config2_shape_subset(orc, p, fast3):
+    """n=200 m=852 3-SAT, through the register-resident kernel (fast3=1) and the
+    general one (fast3=0)."""
+    print(f"[eval/bcp] n=200 m=852 N=96 A=64 p={p} fast3={fast3}", flush=True)
     b = W.random_3sat_batch(96, 200, 852, seed=2)
     rows = W.random_assignments(b, 64, p, seed=int(p * 1000))
-    counts, conflict, props = check_batch(orc, b, rows, 64)
+    with G.tuned(eval_fast3=fast3):
+        counts, conflict, props = check_batch(orc, b, rows, 64)
     if p <= 0.05:
         assert conflict.mean() < 0.5 and props.sum() > 0
+
+
+@pytest.mark.parametrize("A", [1, 33, 64])
+def test_fast3_limits(orc, A):
+    """Batches at the register-resident kernel's limits: n=256 (16 words per row),
+    m=1024 (16 full clause groups), a clause-free instance, repeated and complementary
+    literals inside a clause, unit chains that cross groups."""
+    print(f"[eval/bcp] 3-literal kernel limits A={A}", flush=True)
+    from gpupsat_amd.formula import Formula
+    fs = [W.random_3sat_batch(1, 256, 1024, seed=9).instance(0),
+          W.random_3sat_batch(1, 100, 430, seed=10).instance(0),
+          Formula.from_clauses(5, []),
+          Formula.from_clauses(4, [[1, 1, 3], [0, 1, 5], [2, 2, 6], [7, 7, 7]]),
+          Formula.from_clauses(1, [[1, 1, 1]]),
+          W.random_3sat_batch(1, 256, 700, seed=11).instance(0)]
+    # an implication chain x0 -> x1 -> ... -> x199 through clauses (~x_i | x_{i+1} | z),
+    # z = x200 false in every row; the clauses are listed last-link first so each
+    # Gauss-Seidel pass over the groups advances the chain by one group at most
+    chain = [[2 * i, 2 * i + 3, 401] for i in range(198, -1, -1)]
+    fs.append(Formula.from_clauses(201, chain))
+    b = Batch.from_formulas(fs)
+    rng = np.random.default_rng(A)
+    rows = []
+    for k, f in enumerate(fs):
+        n = f.n_vars
+        t = rng.random((A, n)) < 0.02
+        fl = (rng.random((A, n)) < 0.02) & ~t
+        if k == len(fs) - 1:
+            t[:] = False
+            fl[:] = False
+            t[:, 0] = True
+            fl[:, 200] = True
+        rows.append(pack_rows(t, fl).reshape(-1))
+    rows = np.concatenate(rows)
+    for fast3 in (1, 0):
+        with G.tuned(eval_fast3=fast3):
+            _, conflict, props = check_batch(orc, b, rows, A)
+    assert (props[-1] == 199).all() and (conflict[-1] == 0).all()
 
 
 @pytest.mark.parametrize("A", [1, 7, 33, 64])
