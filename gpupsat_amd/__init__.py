@@ -220,14 +220,22 @@ def cubes(vars_) -> np.ndarray:
 
 
 def cube_jobs(batch: Batch, vars_per_inst) -> tuple:
-    """Job arrays (job_inst, cube_off, cube_lits) splitting instance i on vars_per_inst[i]."""
-    ji, co, cl = [], [0], []
+    """Job arrays (job_inst, cube_off, cube_lits) splitting instance i on vars_per_inst[i]
+    into its 2^d cubes, in the reference's JobChooser order (gpsat_cubes)."""
+    ji, cl, lens = [], [], []
     for i, vs in enumerate(vars_per_inst):
-        for cube in cubes(vs):
-            ji.append(i)
-            cl.extend(cube.tolist())
-            co.append(len(cl))
-    return (np.array(ji, dtype=np.int32), np.array(co, dtype=np.int32), np.array(cl or [0], dtype=np.int32))
+        c = cubes(vs)
+        ji.append(np.full(c.shape[0], i, dtype=np.int32))
+        cl.append(c.reshape(-1))
+        lens.append(np.full(c.shape[0], c.shape[1], dtype=np.int64))
+    ji = np.concatenate(ji) if ji else np.zeros(0, dtype=np.int32)
+    lens = np.concatenate(lens) if lens else np.zeros(0, dtype=np.int64)
+    co = np.zeros(len(ji) + 1, dtype=np.int64)
+    np.cumsum(lens, out=co[1:])
+    if co[-1] > np.iinfo(np.int32).max:
+        raise GpsatError("cube literal arena exceeds 2^31 entries")
+    cl = np.concatenate(cl).astype(np.int32) if cl and co[-1] else np.zeros(1, dtype=np.int32)
+    return ji, co.astype(np.int32), cl
 
 
 class SolveOpts(ctypes.Structure):
@@ -239,6 +247,7 @@ class SolveOpts(ctypes.Structure):
         ("phase_init", ctypes.c_int32), ("max_ms", ctypes.c_double), ("waves", ctypes.c_int32),
         ("clause_mem", ctypes.c_int32), ("verify", ctypes.c_int32), ("chunk", ctypes.c_int32),
         ("epoch_conflicts", ctypes.c_int64), ("epoch_growth", ctypes.c_float), ("max_epochs", ctypes.c_int32),
+        ("deadline_ms", ctypes.c_double), ("inst_conflicts", ctypes.c_int64), ("job_order", ctypes.c_int32),
     ]
 
     @staticmethod
@@ -320,6 +329,37 @@ class GpuBatch:
     def eval_bcp_dev(self, rows: DevBuf, A: int, counts: DevBuf, out: DevBuf, conflict: DevBuf, props: DevBuf):
         _check(lib().gpsat_eval_bcp_dev(self.h, rows.ptr, ctypes.c_int32(A), counts.ptr, out.ptr, conflict.ptr,
                                         props.ptr))
+
+    # ---- autotuner
+    def autotune_solve(self, opts: Optional["SolveOpts"] = None, jobs=None, sample_jobs: int = 0,
+                       trial_ms: float = 0.0):
+        """Launch shape (waves, clause_mem, chunk) for this batch and job list from short
+        trial solves (gpsat_autotune_solve).  Returns (SolveOpts, report text)."""
+        o = opts if opts is not None else SolveOpts.default()
+        best = SolveOpts()
+        rep = ctypes.create_string_buffer(8192)
+        if jobs is None:
+            ji = co = cl = None
+            nj = 0
+        else:
+            ji = np.ascontiguousarray(jobs[0], dtype=np.int32)
+            co = np.ascontiguousarray(jobs[1], dtype=np.int32)
+            cl = np.ascontiguousarray(jobs[2], dtype=np.int32)
+            nj = len(ji)
+        _check(lib().gpsat_autotune_solve(self.h, ctypes.byref(o), _p(ji) if ji is not None else None,
+                                          _p(co) if co is not None else None, _p(cl) if cl is not None else None,
+                                          ctypes.c_int32(nj), ctypes.c_int32(sample_jobs), ctypes.c_double(trial_ms),
+                                          ctypes.byref(best), rep, ctypes.c_int32(len(rep))))
+        return best, rep.value.decode()
+
+    def autotune_eval(self, rows: "DevBuf", A: int, do_eval: bool = True, do_bcp: bool = True, reps: int = 0) -> str:
+        """Time the eval/BCP launch shapes on these rows and keep the fastest
+        (gpsat_autotune_eval).  Returns the report text."""
+        rep = ctypes.create_string_buffer(8192)
+        _check(lib().gpsat_autotune_eval(self.h, rows.ptr, ctypes.c_int32(A), ctypes.c_int32(int(do_eval)),
+                                         ctypes.c_int32(int(do_bcp)), ctypes.c_int32(reps), rep,
+                                         ctypes.c_int32(len(rep))))
+        return rep.value.decode()
 
     # ---- solve path
     def solve(self, opts: Optional[SolveOpts] = None, jobs=None, act_init=None, phase_init=None,

@@ -4,6 +4,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <chrono>
 #include <numeric>
 #include <string>
 #include <vector>
@@ -74,6 +75,9 @@ int gpsat_solve_opts_default(gpsat_solve_opts* o)
     o->epoch_conflicts = 0;
     o->epoch_growth = 2.0f;
     o->max_epochs = 16;
+    o->deadline_ms = 0.0;
+    o->inst_conflicts = 0;
+    o->job_order = 0;
     return GPSAT_OK;
 }
 
@@ -130,12 +134,23 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     }
     std::vector<int32_t> ncubes(N, 0);
     for (const JobDesc& j : jobs) ncubes[j.inst]++;
-    // largest instances first: the persistent grid then ends on short jobs
-    std::stable_sort(jobs.begin(), jobs.end(), [&](const JobDesc& x, const JobDesc& y) {
-        const int64_t cx = (int64_t)d.h_meta[x.inst].n * (d.h_coff[d.h_meta[x.inst].coff_base + d.h_meta[x.inst].m] + 1);
-        const int64_t cy = (int64_t)d.h_meta[y.inst].n * (d.h_coff[d.h_meta[y.inst].coff_base + d.h_meta[y.inst].m] + 1);
-        return cx > cy;
-    });
+    // queue order by instance size n * (L + 1): largest first lets the persistent grid end
+    // on short jobs; smallest first decides the most instances before a deadline.  Ties
+    // keep the caller's order, so the cubes of one instance stay adjacent.
+    if (o.job_order < 0 || o.job_order > 2) {
+        set_error("job_order must be 0, 1 or 2");
+        return GPSAT_ERR_ARG;
+    }
+    if (o.job_order != 2) {
+        std::vector<int64_t> cost(N);
+        for (int32_t i = 0; i < N; i++)
+            cost[i] = (int64_t)d.h_meta[i].n * (d.h_coff[d.h_meta[i].coff_base + d.h_meta[i].m] + 1);
+        const bool desc = o.job_order == 0;
+        std::stable_sort(jobs.begin(), jobs.end(), [&](const JobDesc& x, const JobDesc& y) {
+            if (cost[x.inst] != cost[y.inst]) return desc ? cost[x.inst] > cost[y.inst] : cost[x.inst] < cost[y.inst];
+            return x.inst < y.inst;
+        });
+    }
     std::vector<int64_t> var_base(N + 1, 0);
     for (int32_t i = 0; i < N; i++) var_base[i + 1] = var_base[i] + d.h_meta[i].n;
     int32_t max_L = 0;
@@ -157,6 +172,8 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     if (hipDeviceGetAttribute(&clock_khz, hipDeviceAttributeWallClockRate, d.device) != hipSuccess || clock_khz <= 0)
         clock_khz = 100000;
     a.o.max_ticks = o.max_ms > 0 ? (int64_t)(o.max_ms * clock_khz) : 0;
+    a.o.deadline_ticks = o.deadline_ms > 0 ? (int64_t)(o.deadline_ms * clock_khz) : 0;
+    a.o.inst_conflicts = o.inst_conflicts;
     a.max_n = d.max_n;
     a.wo_cap = max_L;
     a.wl_cap = 2 * d.max_n * a.o.watch_slack;
@@ -203,6 +220,9 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     HIP_OR(mem.alloc(d_jobs, jobs.size()));
     HIP_OR(mem.alloc(d_cubes, cubes.size()));
     HIP_OR(mem.alloc(d_head, 1));
+    unsigned long long *d_tstart, *d_iconf;
+    HIP_OR(mem.alloc(d_tstart, 1));
+    HIP_OR(mem.alloc(d_iconf, N));
     HIP_OR(mem.alloc(d_status, N));
     HIP_OR(mem.alloc(d_refuted, N));
     HIP_OR(mem.alloc(d_undef, N));
@@ -224,6 +244,8 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     HIP_OR(hipMemcpyAsync(d_jobs, jobs.data(), sizeof(JobDesc) * jobs.size(), hipMemcpyHostToDevice, st));
     HIP_OR(hipMemcpyAsync(d_cubes, cubes.data(), sizeof(int32_t) * cubes.size(), hipMemcpyHostToDevice, st));
     HIP_OR(hipMemsetAsync(d_head, 0, sizeof(int32_t), st));
+    HIP_OR(hipMemsetAsync(d_tstart, 0, sizeof(unsigned long long), st));
+    HIP_OR(hipMemsetAsync(d_iconf, 0, sizeof(unsigned long long) * N, st));
     HIP_OR(hipMemcpyAsync(d_status, init_status.data(), sizeof(int32_t) * N, hipMemcpyHostToDevice, st));
     HIP_OR(hipMemsetAsync(d_refuted, 0, sizeof(int32_t) * N, st));
     HIP_OR(hipMemsetAsync(d_undef, 0, sizeof(int32_t) * N, st));
@@ -237,6 +259,7 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     a.lits = d.lits;
     a.coff = d.coff;
     a.jobs = d_jobs;
+    a.t_start = d_tstart;
     a.cube_lits = d_cubes;
     a.n_jobs = n_jobs;
     a.n_inst = N;
@@ -247,6 +270,7 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     a.inst_status = d_status;
     a.inst_refuted = d_refuted;
     a.inst_undef = d_undef;
+    a.inst_conf = d_iconf;
     a.inst_ncubes = d_ncubes;
     a.model_rows = d_models;
     a.job_stats = d_jstats;
@@ -334,12 +358,16 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
         }
         epochs_run = 0;
         hipError_t le = hipSuccess;
+        const auto t_host = std::chrono::steady_clock::now();
         for (int32_t e = 0; e < o.max_epochs; e++) {
             le = hipGraphLaunch(exec, st);
             if (le == hipSuccess) le = hipStreamSynchronize(st);
             if (le != hipSuccess) break;
             epochs_run++;
             if (*h_remaining == 0) break;
+            // past the deadline every further epoch would only mark its jobs UNDEF
+            const double el = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t_host).count();
+            if (o.deadline_ms > 0 && el > o.deadline_ms) break;
         }
         (void)hipGraphExecDestroy(exec);
         (void)hipGraphDestroy(graph);

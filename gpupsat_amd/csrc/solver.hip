@@ -97,6 +97,8 @@ struct Ctx {
     int32_t base_trail;      // trail entries of the formula's unit clauses
     int32_t setup_inst;      // instance whose watch lists are built (-1: none)
     bool setup_ok;
+    bool ran;            // the current job set up its instance (its per-var state is valid)
+    int64_t conf_reported;  // conflicts of the current job already added to its instance pool
     int64_t conflicts, decisions, props, learned, restarts;
     int32_t levelmax;
 };
@@ -692,13 +694,29 @@ __device__ void reset_instance(Ctx& J)
     J.larena_used = 0;
 }
 
+// Adds the conflicts of this job not yet reported to its instance's pool; returns the
+// pool's new total (wave-uniform).
+__device__ int64_t report_conflicts(Ctx& J, const SolveArgs& a, int32_t inst)
+{
+    unsigned long long tot = 0;
+    const unsigned long long delta = (unsigned long long)(J.conflicts - J.conf_reported);
+    if (lane_id() == 0) tot = atomicAdd(&a.inst_conf[inst], delta) + delta;
+    J.conf_reported = J.conflicts;
+    return (int64_t)__shfl(tot, 0);
+}
+
 // Returns the job verdict: 0 SAT, 1 UNSAT, 2 UNDEF (budget), 3 skipped (instance decided).
-__device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_t max_conflicts)
+// deadline: wall-clock tick after which the job stops as UNDEF (0: none).
+__device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_t max_conflicts, uint64_t deadline)
 {
     const SolveOptsDev& o = a.o;
     J.conflicts = J.decisions = J.props = J.learned = J.restarts = 0;
     J.levelmax = 0;
+    J.ran = false;
+    J.conf_reported = 0;
     if (load_status(&a.inst_status[jd.inst]) != 2) return 3;
+    if (o.inst_conflicts > 0 && (int64_t)__atomic_load_n(&a.inst_conf[jd.inst], __ATOMIC_RELAXED) >= o.inst_conflicts)
+        return 2;
     const uint64_t t_start = wall_clock64();
     if (J.setup_inst == jd.inst) {
         reset_instance(J);
@@ -706,6 +724,7 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
         J.setup_ok = setup_instance(J, a, jd.inst);
         J.setup_inst = jd.inst;
     }
+    J.ran = true;
     if (!J.setup_ok) return 1;
     const int32_t m = J.m;
     bool unsat = false;
@@ -760,7 +779,12 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
             J.inc *= decay_inv;
             if (J.inc > 1e30f) rescale_activity(J);
             if (max_conflicts > 0 && J.conflicts >= max_conflicts) return 2;
-            if (o.max_ticks > 0 && (J.conflicts & 63) == 0 && (int64_t)(wall_clock64() - t_start) > o.max_ticks) return 2;
+            if ((J.conflicts & 63) == 0) {
+                const uint64_t now = wall_clock64();
+                if (o.max_ticks > 0 && (int64_t)(now - t_start) > o.max_ticks) return 2;
+                if (deadline && now > deadline) return 2;
+                if (o.inst_conflicts > 0 && report_conflicts(J, a, jd.inst) >= o.inst_conflicts) return 2;
+            }
         } else {
             if (restart_cnt >= restart_thr) {
                 restart_thr = (int32_t)((float)restart_thr * o.restart_factor);
@@ -777,7 +801,9 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
             if (o.max_decisions > 0 && J.decisions > o.max_decisions) return 2;
             if ((J.decisions & 1023) == 0) {
                 if (load_status(&a.inst_status[jd.inst]) != 2) return 3;
-                if (o.max_ticks > 0 && (int64_t)(wall_clock64() - t_start) > o.max_ticks) return 2;
+                const uint64_t now = wall_clock64();
+                if (o.max_ticks > 0 && (int64_t)(now - t_start) > o.max_ticks) return 2;
+                if (deadline && now > deadline) return 2;
             }
             J.dl++;
             if (J.dl > J.levelmax) J.levelmax = J.dl;
@@ -791,7 +817,7 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
 }  // namespace
 
 template <bool kLds>
-__global__ __launch_bounds__(64) void k_solve(SolveArgs a)
+__global__ __launch_bounds__(64, 4) void k_solve(SolveArgs a)
 {
     extern __shared__ __align__(16) char smem[];
     const int lane = lane_id();
@@ -812,6 +838,18 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a)
         order = a.active + (int64_t)(e & 1) * a.n_jobs;
         budget = a.epoch_budget[e];
     }
+    // the whole-solve deadline counts from the first wavefront's start (kept across epochs)
+    uint64_t deadline = 0;
+    if (a.o.deadline_ticks > 0) {
+        unsigned long long t0 = 0;
+        if (lane == 0) {
+            const unsigned long long now = wall_clock64();
+            const unsigned long long old = atomicCAS(a.t_start, 0ull, now);
+            t0 = old ? old : now;
+        }
+        t0 = __shfl(t0, 0);
+        deadline = (uint64_t)t0 + (uint64_t)a.o.deadline_ticks;
+    }
     int32_t j = 0, j_end = 0;
     for (;;) {
         if (j >= j_end) {
@@ -826,10 +864,15 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a)
         const JobDesc jd = a.jobs[order ? order[j] : j];
         j++;
         const uint64_t t_job = wall_clock64();
-        const int32_t st = run_job(J, a, jd, budget);
+        // past the deadline a job is not started: UNDEF
+        J.ran = false;
+        J.conflicts = J.decisions = J.props = J.learned = J.restarts = 0;
+        J.levelmax = 0;
+        const int32_t st = (deadline && t_job > deadline) ? 2 : run_job(J, a, jd, budget, deadline);
         const uint64_t job_ticks = wall_clock64() - t_job;
         __syncthreads();
         const int32_t inst = jd.inst;
+        if (a.o.inst_conflicts > 0 && J.ran) (void)report_conflicts(J, a, inst);
         if (st == 0) {
             int32_t won = 0;
             if (lane == 0) won = atomicCAS(&a.inst_status[inst], 2, 0) == 2;
@@ -853,7 +896,7 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a)
             }
         } else if (st == 2) {
             if (lane == 0) atomicAdd(&a.inst_undef[inst], 1);
-            if (a.act_store) {
+            if (a.act_store && J.ran) {
                 // a job stopped at an epoch boundary restarts next epoch from these
                 const int64_t vb = a.var_base[inst];
                 for (int32_t v = lane; v < J.n; v += kWave) {
@@ -862,7 +905,9 @@ __global__ __launch_bounds__(64) void k_solve(SolveArgs a)
                 }
             }
         }
-        if (a.var_bumps) {
+        // per-var state belongs to this job's instance only if the job set it up (a job
+        // skipped because its instance was decided, or because the deadline passed, did not)
+        if (a.var_bumps && J.ran) {
             const int64_t vb = a.var_base[inst];
             for (int32_t v = lane; v < J.n; v += kWave)
                 if (J.bumps[v]) atomicAdd(&a.var_bumps[vb + v], J.bumps[v]);

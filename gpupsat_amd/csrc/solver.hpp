@@ -21,6 +21,8 @@ struct SolveOptsDev {
     float var_decay;          // VSIDS activity decay per conflict
     int32_t phase_init;       // initial phase when no hint is given: 0 false, 1 true
     int64_t max_ticks;        // per-job wall-clock budget in device clock ticks (<= 0: none) -> UNDEF
+    int64_t deadline_ticks;   // whole-solve budget in ticks after the first wavefront starts (<= 0: none)
+    int64_t inst_conflicts;   // conflict pool of each instance over all its jobs (<= 0: none)
 };
 
 struct JobDesc {
@@ -44,10 +46,12 @@ struct SolveArgs {
     const float* act_init;          // [sum n] initial VSIDS scores (null: zero)
     const int8_t* phase_init;       // [sum n] initial phases 0/1 (null: opts.phase_init)
     int32_t* queue_head;            // device work-queue counter (counts chunks)
+    unsigned long long* t_start;    // wall-clock tick at which the first wavefront started (0: not yet)
     int32_t chunk;                  // consecutive jobs handed out per queue grab
     int32_t* inst_status;           // [n_inst] 2 undecided, 0 SAT, 1 UNSAT
     int32_t* inst_refuted;          // [n_inst] refuted cubes
     int32_t* inst_undef;            // [n_inst] cubes stopped by a budget
+    unsigned long long* inst_conf;  // [n_inst] conflicts spent by the instance's jobs so far
     const int32_t* inst_ncubes;     // [n_inst]
     uint32_t* model_rows;           // [total_words] model of each SAT instance (literal bitmap)
     int64_t* job_stats;             // [n_jobs * JS_COUNT]

@@ -173,6 +173,14 @@ typedef struct gpsat_solve_opts {
                                 conflicts, and a stopped job keeps its activities and phases */
     float epoch_growth;      /* budget growth per epoch (default 2) */
     int32_t max_epochs;      /* epochs before the remaining jobs are UNDEF (default 16) */
+    double deadline_ms;      /* whole-solve wall-clock budget in ms from the first wavefront's
+                                start (<= 0: none): jobs running when it passes stop, jobs not
+                                yet started are not run; both count as UNDEF */
+    int64_t inst_conflicts;  /* per-instance conflict pool shared by all its jobs (<= 0: none):
+                                once spent, its running jobs stop and the rest are skipped (UNDEF) */
+    int32_t job_order;       /* queue order: 0 largest instances first (default: the grid ends on
+                                short jobs), 1 smallest first (decides the most instances before a
+                                deadline), 2 as given; cubes of one instance stay adjacent */
 } gpsat_solve_opts;
 
 typedef struct gpsat_solve_info {
@@ -200,6 +208,23 @@ int gpsat_solve(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t *job_in
                 const int32_t *cube_lits, int32_t n_jobs, const float *act_init, const int8_t *phase_init,
                 int32_t *status, uint32_t *model_rows, int32_t *verified, int64_t *job_stats, int32_t *var_bumps,
                 gpsat_solve_info *info);
+
+/* ---- launch-shape autotuner (no reference counterpart: the reference takes -b/-t and
+ * fixes its strategies at compile time, src/SATSolver/Configs.cuh).
+ * gpsat_autotune_solve: short trial solves of a sample of the jobs (whole instances, at
+ * most sample_jobs jobs; 0: 4096) under each launch shape -- clause part in LDS or HBM,
+ * 4/8/12/16 persistent wavefronts per CU, then 1/8/32/64 jobs per queue grab when there
+ * are cubes -- each with deadline trial_ms (0: 300).  *best = *base with waves,
+ * clause_mem and chunk of the shape with the most propagations per ms of kernel time.
+ * Verdicts do not depend on the shape.  report (optional) receives one line per trial.
+ * gpsat_autotune_eval: times reps launches (0: 10) of clause evaluation and/or BCP on the
+ * caller's rows under each eval/BCP shape ("eval_fast3", "eval_stage_arena",
+ * "eval_waves_per_block") and leaves the tuning keys at the fastest. */
+int gpsat_autotune_solve(gpsat_batch *b, const gpsat_solve_opts *base, const int32_t *job_inst, const int32_t *cube_off,
+                         const int32_t *cube_lits, int32_t n_jobs, int32_t sample_jobs, double trial_ms,
+                         gpsat_solve_opts *best, char *report, int32_t report_len);
+int gpsat_autotune_eval(gpsat_batch *b, const uint32_t *rows_dev, int32_t A, int32_t do_eval, int32_t do_bcp,
+                        int32_t reps, char *report, int32_t report_len);
 
 /* ---- a17: the learned branching network (no reference counterpart).  Message passing
  * over the literal-clause incidence graph of a batch: 64 hidden units, `rounds` rounds,

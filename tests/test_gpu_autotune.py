@@ -1,0 +1,58 @@
+"""The launch-shape autotuner: it picks a shape from timed trials, and the verdicts and
+fixpoints under the chosen shape are the oracle's."""
+import numpy as np
+import pytest
+
+import gpupsat_amd as G
+from gpupsat_amd import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def test_autotune_eval_keeps_parity(orc):
+    print("[autotune] eval/BCP shapes on 512 x n=200 3-SAT", flush=True)
+    b = W.random_3sat_batch(512, 200, 852, seed=21)
+    A = 64
+    rows = W.random_assignments(b, A, 0.1, seed=22)
+    keys = ("eval_fast3", "eval_stage_arena", "eval_waves_per_block")
+    saved = {k: G.tune_get(k) for k in keys}
+    try:
+        with G.GpuBatch(b) as gb:
+            rb = G.DevBuf.from_host(rows)
+            rep = gb.autotune_eval(rb, A, reps=3)
+            assert "chosen:" in rep and rep.count("ms") >= 4
+            counts = gb.clause_eval(rows, A)
+            out, conflict, props = gb.bcp_fixpoint(rows, A)
+        np.testing.assert_array_equal(counts, orc.eval_batch(b, rows, A))
+        e_out, e_conf, e_props = orc.bcp_batch(b, rows, A)
+        np.testing.assert_array_equal(conflict, e_conf)
+        np.testing.assert_array_equal(props, e_props)
+        np.testing.assert_array_equal(out, e_out)
+    finally:
+        for k, v in saved.items():
+            G.tune_set(k, v)
+
+
+def test_autotune_solve_same_verdicts(orc):
+    print("[autotune] solve shapes on mixed k-SAT with cubes", flush=True)
+    b, _ = W.mixed_ksat_batch(16, seed=5)
+    vs = [G.cube_vars(b.instance(i), threads=1, blocks=1) for i in range(b.n_inst)]   # depth 4
+    jobs = G.cube_jobs(b, vs)
+    base = G.SolveOpts.default(max_conflicts=3000, job_order=1)
+    with G.GpuBatch(b) as gb:
+        best, rep = gb.autotune_solve(base, jobs=jobs, trial_ms=150.0)
+        assert "chosen:" in rep
+        assert best.waves > 0 and best.clause_mem in (1, 2)
+        assert best.max_conflicts == 3000 and best.job_order == 1
+        r1 = gb.solve(base, jobs=jobs)
+        r2 = gb.solve(best, jobs=jobs)
+        models = gb.models(r2["model_rows"])
+    # budgets are per job and deterministic, so every decided verdict agrees
+    for x, y in zip(r1["status"].tolist(), r2["status"].tolist()):
+        assert x == y or G.UNDEF in (x, y)
+    for i, x in enumerate(r2["status"].tolist()):
+        if x == G.SAT:
+            assert orc.verify_model(b.instance(i), models[i])
+        elif x == G.UNSAT:
+            st, _, _ = orc.solve_cdcl(b.instance(i), max_conflicts=200_000)
+            assert st != G.SAT

@@ -184,3 +184,51 @@ def test_restart_epochs_graph(orc, cubes):
     exp = [truth(orc, f) for f in fs]
     assert res["status"].tolist() == exp
     assert eager["status"].tolist() == exp
+
+
+@pytest.mark.parametrize("epochs", [False, True])
+def test_deadline_stops_the_launch(orc, epochs):
+    """A whole-solve deadline: the hard PHP(9,8) instances end UNDEF when it passes, the
+    easy ones are decided exactly, and the launch does not outlive the deadline by much."""
+    print(f"[solve] deadline epochs={epochs}", flush=True)
+    fs = [W.pigeonhole_formula(9, 8)] * 6
+    fs += [W.random_3sat_batch(1, 50, 213, seed=300 + i).instance(0) for i in range(40)]
+    b = Batch.from_formulas(fs)
+    kw = dict(deadline_ms=400.0, max_ms=0.0)
+    if epochs:
+        kw.update(epoch_conflicts=500, max_epochs=30)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(**kw))
+        check_models(orc, gb, res, b)
+    st = res["status"].tolist()
+    assert st[:6] == [G.UNDEF] * 6
+    exp = [truth(orc, f) for f in fs[6:]]
+    assert st[6:] == exp
+    assert res["info"]["kernel_ms"] < 3000.0
+
+
+@pytest.mark.parametrize("order", [0, 1, 2])
+def test_instance_conflict_pool_and_order(orc, order):
+    """A conflict pool per instance shared by its cubes: PHP(8,7) spends it and ends UNDEF,
+    the easy instances are decided exactly, and no instance's jobs overrun the pool by
+    more than one check interval (64 conflicts) per job; every queue order gives the same
+    verdicts."""
+    print(f"[solve] instance conflict pool, job order {order}", flush=True)
+    fs = [W.pigeonhole_formula(8, 7)] * 3
+    fs += [W.random_3sat_batch(1, 60, 256, seed=400 + i).instance(0) for i in range(30)]
+    b = Batch.from_formulas(fs)
+    jobs = G.cube_jobs(b, [np.array([0, 5, 9], dtype=np.int32)] * b.n_inst)
+    pool = 3000
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(inst_conflicts=pool, job_order=order, chunk=4), jobs=jobs)
+        check_models(orc, gb, res, b)
+    st = res["status"].tolist()
+    assert st[:3] == [G.UNDEF] * 3
+    exp = [truth(orc, f) for f in fs[3:]]
+    for i, e in enumerate(exp):
+        assert st[3 + i] in (e, G.UNDEF)
+    assert sum(x != G.UNDEF for x in st[3:]) >= 25
+    js = res["job_stats"]
+    ji = np.asarray(jobs[0])
+    for i in range(3):
+        assert js[ji == i, 1].sum() <= pool + 64 * int((ji == i).sum())
