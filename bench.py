@@ -133,6 +133,113 @@ def solve_leg(G, W, b, cpu):
     return leg
 
 
+def log(msg: str):
+    """Progress on stderr; the JSON line alone goes to stdout."""
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def cube_depth_for(n: int) -> int:
+    """Cube depth 8..12 growing with the instance size (n in [100, 400])."""
+    return 8 + min(4, max(0, (n - 100) * 5 // 301))
+
+
+def mixed_leg(G, W, N: int, deadline_ms: float, pool: int):
+    """4,096 mixed random k-SAT (k in {3,4,5} at ratios {4.26, 9.9, 21.1}, n in [100, 400])
+    split by the reference's MaxClauseJobChooser into 2^8..2^12 cubes each, all in one
+    persistent queue; launch shape from the autotuner; restart epochs as a replayed HIP
+    graph; a conflict pool per instance and a whole-run deadline."""
+    t0 = time.perf_counter()
+    b, ks = W.mixed_ksat_batch(N, seed=3)
+    ns = b.n_vars.astype(np.int64)
+    vs = []
+    for i in range(N):
+        d = cube_depth_for(int(ns[i]))
+        # the reference's depth rule: floor(log2(T*B*10)) + 1 = d for T*B below
+        vs.append(G.cube_vars(b.instance(i), threads=[13, 26, 52, 103, 205][d - 8], blocks=1))
+    jobs = G.cube_jobs(b, vs)
+    log(f"mixed: {N} instances, {len(jobs[0])} cube jobs ({time.perf_counter() - t0:.1f} s host)")
+    base = G.SolveOpts.default(inst_conflicts=pool, job_order=1, deadline_ms=deadline_ms, epoch_conflicts=500,
+                               epoch_growth=2.0, max_epochs=8)
+    with G.GpuBatch(b) as gb:
+        t1 = time.perf_counter()
+        best, rep = gb.autotune_solve(base, jobs=jobs, sample_jobs=4096, trial_ms=150.0)
+        tune_s = time.perf_counter() - t1
+        log(f"mixed: autotune {tune_s:.1f} s -> waves {best.waves} mem {best.clause_mem} chunk {best.chunk}")
+        res = gb.solve(best, jobs=jobs, want_models=False)
+    st, js, inf = res["status"], res["job_stats"], res["info"]
+    decided = int((st != 2).sum())
+    ms = inf["kernel_ms"]
+    per_k = {str(k): {"instances": int((ks == k).sum()), "sat": int(((st == 0) & (ks == k)).sum()),
+                      "unsat": int(((st == 1) & (ks == k)).sum())} for k in (3, 4, 5)}
+    log(f"mixed: decided {decided}/{N} in {ms:.0f} ms, {inf['epochs']} epochs")
+    return {
+        "workload": f"{N} mixed random k-SAT (k 3/4/5 at 4.26/9.9/21.1, n 100-400), MaxClauseJobChooser cubes "
+                    f"of depth 8-12 ({len(jobs[0])} jobs), one persistent queue, restart epochs as a HIP graph",
+        "budget": f"{pool} conflicts per instance over its cubes, {deadline_ms / 1e3:.0f} s for the run",
+        "decided": decided, "decided_frac": decided / N, "decided_per_s": decided / (ms * 1e-3),
+        "sat": int((st == 0).sum()), "unsat": int((st == 1).sum()), "per_k": per_k,
+        "sat_models_verified_on_gpu": int(res["verified"].sum()),
+        "kernel_ms": ms, "epochs": inf["epochs"], "jobs": int(len(jobs[0])),
+        "jobs_finished": int((js[:, 0] != 2).sum()),
+        "props_per_s": float(js[:, 3].sum()) / (ms * 1e-3), "conflicts_per_s": float(js[:, 1].sum()) / (ms * 1e-3),
+        "launch_shape": {"waves": inf["waves"], "clause_mem": "LDS" if inf["clause_mem"] == 1 else "HBM",
+                         "chunk": inf["chunk"], "autotune_s": tune_s},
+        "autotune_report": rep.strip().splitlines(),
+    }
+
+
+def structured_leg(G, W, n_color: int, n_php: int):
+    """2,000 4-colourings of ER(150, d in [4.2, 4.8]) and 200 PHP(9,8): exact verdicts (every
+    colouring model re-verified on the GPU; PHP(9,8) is UNSAT by the pigeonhole principle)
+    and learned-clause counts."""
+    from gpupsat_amd.formula import Batch
+    t0 = time.perf_counter()
+    col = W.coloring_batch(n_color, seed=4)
+    fs = [col.instance(i) for i in range(n_color)] + [W.pigeonhole_formula(9, 8)] * n_php
+    b = Batch.from_formulas(fs)
+    log(f"structured: {b.n_inst} instances ({time.perf_counter() - t0:.1f} s host)")
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(G.SolveOpts.default(deadline_ms=60000), want_models=False)
+    st, js, inf = res["status"], res["job_stats"], res["info"]
+    c, p = slice(0, n_color), slice(n_color, None)
+    out = {
+        "workload": f"{n_color} 4-colourings of ER(150, d 4.2-4.8) + {n_php} PHP(9,8), one wavefront per instance",
+        "coloring": {"sat": int((st[c] == 0).sum()), "unsat": int((st[c] == 1).sum()), "undef": int((st[c] == 2).sum()),
+                     "models_verified_on_gpu": int(res["verified"][c].sum()),
+                     "learned_clauses_total": int(js[c, 4].sum()), "learned_clauses_mean": float(js[c, 4].mean()),
+                     "conflicts_mean": float(js[c, 1].mean())},
+        "php_9_8": {"unsat": int((st[p] == 1).sum()), "sat": int((st[p] == 0).sum()), "undef": int((st[p] == 2).sum()),
+                    "learned_clauses_total": int(js[p, 4].sum()), "learned_clauses_mean": float(js[p, 4].mean()),
+                    "conflicts_mean": float(js[p, 1].mean()),
+                    "solve_ms_mean": float(js[p, 7].mean() / 1e6)},
+        "verdicts_exact": bool((st[p] == 1).all() and (st[c] != 1).all()
+                               and int(res["verified"][c].sum()) == int((st[c] == 0).sum())),
+        "kernel_ms": inf["kernel_ms"], "instances_per_s": b.n_inst / (inf["kernel_ms"] * 1e-3),
+        "waves": inf["waves"],
+    }
+    log(f"structured: {inf['kernel_ms']:.0f} ms, PHP UNSAT {out['php_9_8']['unsat']}/{n_php}")
+    return out
+
+
+def learned_leg(n: int, cube_ms: float):
+    """Learned split vs plain JobChooser/VSIDS on held-out random 3-SAT (n=300, m=1278):
+    depth-4 cubes from the MaxClause order vs the network's top-scored variables (plus its
+    initial scores and phases), same GPU and budgets; medians over instances both decide."""
+    from gpupsat_amd import train_branching as T
+    from gpupsat_amd import workloads as W
+    if not os.path.exists(T.DEFAULT_WEIGHTS):
+        return {"skipped": "no trained weights"}
+    theta = np.load(T.DEFAULT_WEIGHTS)
+    b = W.random_3sat_batch(n, 300, 1278, seed=777)
+    import gpupsat_amd as G
+    opts = G.SolveOpts.default(max_conflicts=200_000, max_ms=cube_ms)
+    log(f"learned: {n} held-out instances, depth-4 cubes, {cube_ms:.0f} ms per cube")
+    out = T.compare(b, theta, opts, cube_depth=4)
+    out["workload"] = f"{n} held-out random 3-SAT (n=300, m=1278), 16 cubes each"
+    out["weights"] = "gpupsat_amd/branching_weights.npy (trained on the GPU by gpsat_gnn_train_step)"
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -142,7 +249,11 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-instances", type=int, default=20_000)
-    ap.add_argument("--no-solve", action="store_true")
+    ap.add_argument("--no-solve", action="store_true", help="skip every solve leg")
+    ap.add_argument("--no-legs", action="store_true", help="skip the mixed, structured and learned legs")
+    ap.add_argument("--mixed-instances", type=int, default=4096)
+    ap.add_argument("--mixed-deadline-ms", type=float, default=20000.0)
+    ap.add_argument("--learned-instances", type=int, default=256)
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the instance batch would shard across GPUs with no "
@@ -266,7 +377,12 @@ def main():
 
     legs = {}
     if solve_batch is not None:
+        log("solve leg")
         legs["solve"] = solve_leg(G, W, solve_batch, solve_cpu)
+    if not args.no_solve and not args.no_legs:
+        legs["mixed_ksat_cubes"] = mixed_leg(G, W, args.mixed_instances, args.mixed_deadline_ms, 100_000)
+        legs["structured"] = structured_leg(G, W, 2000, 200)
+        legs["learned_split"] = learned_leg(args.learned_instances, 3000.0)
 
     out = {
         "metric": METRIC,

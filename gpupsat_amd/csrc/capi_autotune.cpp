@@ -47,8 +47,9 @@ struct Report {
     }
 };
 
-// The sampled job list: whole instances (all of their jobs, so chunked cube reuse is
-// part of what is measured), every stride-th instance in job order, up to `limit` jobs.
+// The sampled job list: up to 256 instances spread evenly over the job order, and of each
+// a run of consecutive jobs (all of its jobs, or its first limit/instances cubes, so
+// chunked cube reuse is part of what is measured), limit jobs in all.
 struct Sample {
     std::vector<int32_t> inst, off, lits;
 };
@@ -76,11 +77,12 @@ Sample sample_jobs(const int32_t* job_inst, const int32_t* cube_off, const int32
         runs.emplace_back(j, e);
         j = e;
     }
-    const int64_t avg = std::max<int64_t>(1, (int64_t)n_jobs / std::max<size_t>(1, runs.size()));
-    const int32_t want_runs = (int32_t)std::max<int64_t>(1, limit / avg);
-    const size_t stride = std::max<size_t>(1, runs.size() / (size_t)want_runs);
+    const size_t want = std::min<size_t>(runs.size(), std::min<size_t>(256, (size_t)limit));
+    const size_t stride = std::max<size_t>(1, runs.size() / std::max<size_t>(1, want));
+    const int32_t per_run = std::max<int32_t>(1, limit / (int32_t)std::max<size_t>(1, want));
     for (size_t r = 0; r < runs.size() && (int32_t)s.inst.size() < limit; r += stride) {
-        for (int32_t j = runs[r].first; j < runs[r].second; j++) {
+        const int32_t e = std::min(runs[r].second, runs[r].first + per_run);
+        for (int32_t j = runs[r].first; j < e; j++) {
             s.inst.push_back(job_inst[j]);
             for (int32_t k = cube_off[j]; k < cube_off[j + 1]; k++) s.lits.push_back(cube_lits[k]);
             s.off.push_back((int32_t)s.lits.size());
