@@ -361,7 +361,7 @@ def main():
     achieved = bytes_fused / (ms_fused * 1e-3) / 1e9
 
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", "eval_bcp_config2_traffic.json")
+    tpath = os.path.join(ROOT, "profiles", "eval_bcp_3sat_n200_m852_A64_traffic.json")
     if os.path.exists(tpath):
         try:
             traffic = json.load(open(tpath)).get("bytes_per_launch")

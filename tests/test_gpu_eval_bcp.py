@@ -42,7 +42,7 @@ def check_batch(orc, b, rows, A):
 
 @pytest.mark.parametrize("fast3", [1, 0])
 @pytest.mark.parametrize("p", [0.02, 0.05, 0.1, 0.3])
-def test_config2_shape_subset(orc, p, fast3):
+def test_3sat_n200_subset(orc, p, fast3):
     """n=200 m=852 3-SAT, through the register-resident kernel (fast3=1) and the
     general one (fast3=0)."""
     print(f"[eval/bcp] n=200 m=852 N=96 A=64 p={p} fast3={fast3}", flush=True)
@@ -143,10 +143,10 @@ def test_reference_fixtures_full_models(orc):
     counts, _, _ = check_batch(orc, b, rows, A)
 
 
-def test_config2_full_size_properties(orc):
+def test_3sat_n200_full_batch_properties(orc):
     """10,000 x (n=200, m=852), A=64, p=0.3: oracle parity on a 256-instance sample,
     size-independent properties everywhere."""
-    print("[eval/bcp] config-2 full size", flush=True)
+    print("[eval/bcp] 10,000 x n=200 full batch", flush=True)
     N, n, m, A = 10_000, 200, 852, 64
     b = W.random_3sat_batch(N, n, m, seed=0)
     rows = W.random_assignments(b, A, 0.3, seed=0)

@@ -49,8 +49,8 @@ def test_reference_fixtures(orc):
 
 
 @pytest.mark.parametrize("mem", [1, 2])
-def test_config1_verdicts(orc, mem):
-    print(f"[solve] config-1 100 x n=50 m=213 clause_mem={mem}", flush=True)
+def test_3sat_n50_verdicts(orc, mem):
+    print(f"[solve] 100 x n=50 m=213 clause_mem={mem}", flush=True)
     b = W.random_3sat_batch(100, 50, 213, seed=0)
     with G.GpuBatch(b) as gb:
         res = gb.solve(opts(clause_mem=mem))

@@ -270,7 +270,7 @@ def test_cube_var_choice(orc):
     assert len(orc.choose_cube_vars(tiny, 0, 32, 32, 0)) == 3     # 2^(4-2) = 4 -> log2 4 + 1
 
 
-def test_config1_solvers_agree(orc):
+def test_3sat_n50_solvers_agree(orc):
     b = W.random_3sat_batch(100, 50, 213, seed=0)
     n_sat = 0
     for i in range(b.n_inst):
