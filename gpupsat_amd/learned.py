@@ -163,6 +163,29 @@ def train(graphs: Sequence[GnnGraph], theta: Optional[np.ndarray] = None, epochs
     return out
 
 
+def train_sets(sets, theta: Optional[np.ndarray] = None, epochs: int = 10, lr: float = 2e-3, alpha: float = 1.0,
+               log=None) -> np.ndarray:
+    """Adam over labelled batches [(GpuBatch, (target, phase, mask)), ...], building each
+    batch's graph (incidence structure + activations) only for its own step, so device
+    memory holds one batch's activations at a time however many batches there are."""
+    model = GnnModel(theta if theta is not None else init_params(0))
+    history = []
+    for ep in range(epochs):
+        losses = []
+        for gb, lab in sets:
+            g = GnnGraph(gb)
+            g.labels(*lab)
+            losses.append(g.train_step(model, lr, alpha))
+            g.close()
+        history.append(float(np.mean(losses)))
+        if log:
+            log(f"epoch {ep}: loss {history[-1]:.5f}")
+    out = model.params()
+    model.close()
+    train_sets.history = history
+    return out
+
+
 def propose(model: GnnModel, gb: GpuBatch, rounds: int = ROUNDS):
     """Network outputs for a batch: (act_init, phase_init, scores per instance)."""
     g = GnnGraph(gb, rounds)

@@ -1,7 +1,7 @@
 """Train the learned branching network on labels from the plain solver, then compare
 the solver with and without its proposals on held-out instances.
 
-    python -m gpupsat_amd.train_branching --train 8192 --epochs 20 --out gpupsat_amd/branching_weights.npy
+    python -m gpupsat_amd.train_branching --train 50000 --epochs 25 --out gpupsat_amd/branching_weights.npy
     python -m gpupsat_amd.train_branching --eval-only --weights gpupsat_amd/branching_weights.npy
 
 Training instances: random 3-SAT, n uniform in [100, 300], ratio uniform in [4.2, 4.3].
@@ -30,10 +30,12 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def build_training_graphs(n_train: int, chunk: int, seed: int, budget_conflicts: int, budget_ms: float):
-    graphs, keep = [], []
+def build_training_sets(n_train: int, chunk: int, seed: int, budget_conflicts: int, budget_ms: float):
+    """Label n_train random 3-SAT instances with the plain solver, chunk instances per
+    batch.  Returns ([(GpuBatch, labels)], stats)."""
+    sets = []
     t0 = time.perf_counter()
-    stats = {"sat": 0, "unsat": 0, "undef": 0}
+    stats = {"instances": n_train, "sat": 0, "unsat": 0, "undef": 0}
     for s in range(0, n_train, chunk):
         b = W.training_3sat_batch(min(chunk, n_train - s), seed=seed * 100_000 + s)
         gb, res, lab = Lr.generate_labels(b, SolveOpts.default(max_conflicts=budget_conflicts, max_ms=budget_ms))
@@ -41,12 +43,10 @@ def build_training_graphs(n_train: int, chunk: int, seed: int, budget_conflicts:
         stats["sat"] += int((st == 0).sum())
         stats["unsat"] += int((st == 1).sum())
         stats["undef"] += int((st == 2).sum())
-        g = Lr.GnnGraph(gb)
-        g.labels(*lab)
-        graphs.append(g)
-        keep.append(gb)
+        sets.append((gb, lab))
+        log(f"labels: {s + b.n_inst}/{n_train} instances, {time.perf_counter() - t0:.0f} s")
     stats["label_seconds"] = time.perf_counter() - t0
-    return graphs, keep, stats
+    return sets, stats
 
 
 def _per_instance(res, job_inst, n_inst):
@@ -150,16 +150,16 @@ def main():
     if a.eval_only:
         theta = np.load(a.weights)
     else:
-        graphs, keep, st = build_training_graphs(a.train, a.chunk, a.seed, a.label_conflicts, a.label_ms)
+        sets, st = build_training_sets(a.train, a.chunk, a.seed, a.label_conflicts, a.label_ms)
         log("labels:", st)
         t0 = time.perf_counter()
-        theta = Lr.train(graphs, Lr.init_params(a.seed), epochs=a.epochs, lr=a.lr, log=log)
+        theta = Lr.train_sets(sets, Lr.init_params(a.seed), epochs=a.epochs, lr=a.lr, log=log)
         st["train_seconds"] = time.perf_counter() - t0
-        st["loss_history"] = Lr.train.history
+        st["loss_history"] = Lr.train_sets.history
         np.save(a.out, theta)
         report["training"] = st
-        for g in graphs:
-            g.close()
+        for gb, _ in sets:
+            gb.close()
     report["evaluation"] = evaluate(theta, a.eval_random, a.eval_color, a.eval_conflicts, a.eval_ms, a.cube_depth)
     print(json.dumps(report))
     if a.report:
