@@ -330,6 +330,26 @@ class GpuBatch:
         _check(lib().gpsat_eval_bcp_dev(self.h, rows.ptr, ctypes.c_int32(A), counts.ptr, out.ptr, conflict.ptr,
                                         props.ptr))
 
+    # ---- device preprocessing
+    def preprocess(self):
+        """The reference's preprocessing (duplicate/tautology removal, unit propagation)
+        for every instance on the device (gpsat_preprocess_batch).  Returns
+        (status int32[N], solved literal-bitmap rows, Batch of the remaining formulas)."""
+        b = self.batch
+        N = b.n_inst
+        status = np.zeros(N, dtype=np.int32)
+        solved = np.zeros(max(self.row_words(1), 1), dtype=np.uint32)
+        L = max(len(b.lits), 1)
+        out_lits = np.zeros(L, dtype=np.int32)
+        out_off = np.zeros(int(b.inst_clause_off[-1]) + 1, dtype=np.int64)
+        out_ioff = np.zeros(N + 1, dtype=np.int64)
+        _check(lib().gpsat_preprocess_batch(self.h, _p(status), _p(solved), _p(out_lits), _p(out_off), _p(out_ioff)))
+        from .formula import Batch as _Batch
+        nc = int(out_ioff[-1])
+        rest = _Batch(out_lits[: out_off[nc]].copy(), out_off[: nc + 1].copy(), out_ioff.copy(),
+                      b.n_vars.copy())
+        return status, solved, rest
+
     # ---- autotuner
     def autotune_solve(self, opts: Optional["SolveOpts"] = None, jobs=None, sample_jobs: int = 0,
                        trial_ms: float = 0.0):

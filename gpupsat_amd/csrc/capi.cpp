@@ -340,6 +340,48 @@ struct DevBuf {
 };
 }  // namespace
 
+int gpsat_preprocess_batch(gpsat_batch* b, int32_t* status, uint32_t* solved_rows, int32_t* out_lits,
+                           int64_t* out_clause_off, int64_t* out_inst_clause_off)
+{
+    ARG_CHECK(b, "null batch");
+    ARG_CHECK(status && solved_rows && out_lits && out_clause_off && out_inst_clause_off, "null output buffer");
+    DeviceBatch& d = b->d;
+    const int32_t N = d.n_inst;
+    out_inst_clause_off[0] = 0;
+    out_clause_off[0] = 0;
+    if (N == 0) return GPSAT_OK;
+    const size_t L = (size_t)std::max<int64_t>(d.n_lits, 1), C = (size_t)d.n_clauses + N;
+    DevBuf dl, dn, ol, oc, om, ds, dsol;
+    HIP_TRY(hipMalloc(&dl.p, L * 4));
+    HIP_TRY(hipMalloc(&dn.p, C * 4));
+    HIP_TRY(hipMalloc(&ol.p, L * 4));
+    HIP_TRY(hipMalloc(&oc.p, C * 4));
+    HIP_TRY(hipMalloc(&om.p, (size_t)N * 4));
+    HIP_TRY(hipMalloc(&ds.p, (size_t)N * 4));
+    HIP_TRY(hipMalloc(&dsol.p, (size_t)std::max<int64_t>(d.total_words, 1) * 4));
+    HIP_TRY(launch_preprocess(d, (int32_t*)dl.p, (int32_t*)dn.p, (int32_t*)ol.p, (int32_t*)oc.p, (int32_t*)om.p,
+                              (int32_t*)ds.p, (uint32_t*)dsol.p, d.stream));
+    std::vector<int32_t> h_ol(L), h_oc(C), h_om(N);
+    HIP_TRY(hipMemcpyAsync(h_ol.data(), ol.p, L * 4, hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipMemcpyAsync(h_oc.data(), oc.p, C * 4, hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipMemcpyAsync(h_om.data(), om.p, (size_t)N * 4, hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipMemcpyAsync(status, ds.p, (size_t)N * 4, hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipMemcpyAsync(solved_rows, dsol.p, (size_t)d.total_words * 4, hipMemcpyDeviceToHost, d.stream));
+    HIP_TRY(hipStreamSynchronize(d.stream));
+    // per-instance compacted regions -> one global CSR (gpsat_batch_create's layout)
+    int64_t nc = 0, nl = 0;
+    for (int32_t i = 0; i < N; i++) {
+        const InstMeta& mi = d.h_meta[i];
+        const int32_t* oci = h_oc.data() + mi.coff_base;
+        for (int32_t c = 0; c < h_om[i]; c++) {
+            for (int32_t k = oci[c]; k < oci[c + 1]; k++) out_lits[nl++] = h_ol[mi.lit_base + k];
+            out_clause_off[++nc] = nl;
+        }
+        out_inst_clause_off[i + 1] = nc;
+    }
+    return GPSAT_OK;
+}
+
 int gpsat_clause_eval(gpsat_batch* b, const uint32_t* rows, int32_t A, int32_t* counts)
 {
     ARG_CHECK(b, "null batch");

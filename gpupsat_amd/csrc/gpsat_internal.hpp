@@ -53,6 +53,9 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
 
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
 
+hipError_t launch_preprocess(const DeviceBatch& b, int32_t* dlits, int32_t* dlen, int32_t* out_lits, int32_t* out_coff,
+                             int32_t* out_m, int32_t* status, uint32_t* solved, hipStream_t st);
+
 }  // namespace gpsat
 
 // The opaque handle behind the C ABI's gpsat_batch.

@@ -126,6 +126,16 @@ int gpsat_formula_free(gpsat_formula *f);
 int gpsat_preprocess(const int32_t *lits, const int64_t *clause_off, int32_t m, int32_t n_vars, int32_t *out_lits,
                      int64_t *out_clause_off, int32_t *out_m, int32_t *solved, int32_t *n_solved, int32_t *status);
 
+/* The same preprocessing for a whole batch on the device, one wavefront per instance
+ * (no reference counterpart as a batch; semantics of FormulaData.cu:82-105 per instance).
+ * Outputs (host buffers): status[n_inst] (0 SAT, 1 UNSAT, 2 UNDEF); solved_rows
+ * (gpsat_batch_row_words(b, 1) words: the solved literals of each instance as a literal
+ * bitmap, zero when it had no unit clause); the remaining formulas in gpsat_batch_create's
+ * layout: out_lits[<= n_lits], out_clause_off[<= n_clauses + 1] (int64, global),
+ * out_inst_clause_off[n_inst + 1]; an UNSAT instance keeps no clauses. */
+int gpsat_preprocess_batch(gpsat_batch *b, int32_t *status, uint32_t *solved_rows, int32_t *out_lits,
+                           int64_t *out_clause_off, int64_t *out_inst_clause_off);
+
 /* ---- a14: cube generation, the JobChooser surface (src/JobsManager/JobChooser.cuh:21-31).
  * strategy 0: MaxClauseJobChooser distributing jobs per thread, 1: MaxClauseJobChooser
  * uniform (src/JobsManager/JobChooser.cu:52-133, VariableChooser.cu:23-40); 2:
