@@ -119,6 +119,13 @@ def solve_leg(G, W, b, cpu):
         "props_per_s": float(js[:, 3].sum()) / (inf["kernel_ms"] * 1e-3),
         "waves": inf["waves"], "clause_mem": "LDS" if inf["clause_mem"] == 1 else "HBM",
     }
+    # per-phase split of the device time (telemetry run on a 2,000-instance slice)
+    sub = b.subset(range(min(2000, N)))
+    with G.GpuBatch(sub) as gt:
+        rt = gt.solve(G.SolveOpts.default(max_ms=30000, telemetry=1), want_models=False)
+    ph = rt["job_stats"][:, 8:13].sum(axis=0).astype(np.float64)
+    leg["phase_split"] = {k: float(v / max(ph.sum(), 1.0)) for k, v in
+                          zip(("setup", "bcp", "analyze", "backtrack", "decide"), ph)}
     if cpu is not None:
         rate, ns, dt, cres, cores = cpu
         agree_cpu = sum(int(cres[i] == st[i]) for i in range(ns) if cres[i] != 2)

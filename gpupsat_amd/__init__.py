@@ -248,6 +248,7 @@ class SolveOpts(ctypes.Structure):
         ("clause_mem", ctypes.c_int32), ("verify", ctypes.c_int32), ("chunk", ctypes.c_int32),
         ("epoch_conflicts", ctypes.c_int64), ("epoch_growth", ctypes.c_float), ("max_epochs", ctypes.c_int32),
         ("deadline_ms", ctypes.c_double), ("inst_conflicts", ctypes.c_int64), ("job_order", ctypes.c_int32),
+        ("telemetry", ctypes.c_int32),
     ]
 
     @staticmethod
@@ -274,7 +275,8 @@ class SolveInfo(ctypes.Structure):
         return {k: getattr(self, k) for k, _ in self._fields_}
 
 
-JOB_STAT_FIELDS = ("status", "conflicts", "decisions", "props", "learned", "restarts", "max_level", "time_ns")
+JOB_STAT_FIELDS = ("status", "conflicts", "decisions", "props", "learned", "restarts", "max_level", "time_ns",
+                   "setup_ns", "bcp_ns", "analyze_ns", "backtrack_ns", "decide_ns")
 
 
 class GpuBatch:
@@ -398,7 +400,7 @@ class GpuBatch:
             co = np.ascontiguousarray(jobs[1], dtype=np.int32)
             cl = np.ascontiguousarray(jobs[2] if len(jobs[2]) else np.zeros(1, np.int32), dtype=np.int32)
             J = len(ji)
-        js = np.zeros((max(J, 1), 8), dtype=np.int64)
+        js = np.zeros((max(J, 1), len(JOB_STAT_FIELDS)), dtype=np.int64)
         nv = int(self.batch.n_vars.astype(np.int64).sum())
         bumps = np.zeros(max(nv, 1), dtype=np.int32) if want_bumps else None
         act = None if act_init is None else np.ascontiguousarray(act_init, dtype=np.float32)

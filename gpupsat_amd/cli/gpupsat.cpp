@@ -264,7 +264,7 @@ int main(int argc, char** argv)
         }
         int32_t st = 2, verified = 0;
         std::vector<uint32_t> rows((2 * (size_t)n + 31) / 32 + 1);
-        std::vector<int64_t> js((size_t)std::max(n_jobs, 1) * 8);
+        std::vector<int64_t> js((size_t)std::max(n_jobs, 1) * GPSAT_JOB_STATS);
         gpsat_solve_info info;
         check(gpsat_solve(b, &so, sequential ? nullptr : ji.data(), sequential ? nullptr : co.data(),
                           sequential ? nullptr : cl.data(), sequential ? 0 : n_jobs, nullptr, nullptr, &st,
@@ -273,7 +273,8 @@ int main(int argc, char** argv)
         gpu_ms = info.kernel_ms + info.verify_ms;
         if (!sequential) {
             int solved_jobs = 0;
-            for (int32_t j = 0; j < n_jobs; j++) solved_jobs += js[(size_t)j * 8] == 0 || js[(size_t)j * 8] == 1;
+            for (int32_t j = 0; j < n_jobs; j++)
+                solved_jobs += js[(size_t)j * GPSAT_JOB_STATS] == 0 || js[(size_t)j * GPSAT_JOB_STATS] == 1;
             printf("Kernel was invoked 1 times\n");
             printf("Jobs size = %d\n", n_jobs);
             printf("There were %d jobs created.\nThere were %d solved jobs\n", n_jobs, solved_jobs);

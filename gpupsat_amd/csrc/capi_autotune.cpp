@@ -125,7 +125,7 @@ int gpsat_autotune_solve(gpsat_batch* b, const gpsat_solve_opts* base, const int
         return GPSAT_ERR_NODEVICE;
     }
     std::vector<int32_t> status(N);
-    std::vector<int64_t> stats((size_t)J * 8);
+    std::vector<int64_t> stats((size_t)J * GPSAT_JOB_STATS);
     Report rep;
     rep.line("autotune solve: %d sampled jobs, trial deadline %.0f ms", J, trial_ms);
 
@@ -147,8 +147,8 @@ int gpsat_autotune_solve(gpsat_batch* b, const gpsat_solve_opts* base, const int
         }
         int64_t props = 0, done = 0;
         for (int32_t j = 0; j < J; j++) {
-            props += stats[(size_t)j * 8 + 3];
-            done += stats[(size_t)j * 8] != 2;
+            props += stats[(size_t)j * GPSAT_JOB_STATS + 3];
+            done += stats[(size_t)j * GPSAT_JOB_STATS] != 2;
         }
         const double rate = inf.kernel_ms > 0 ? (double)props / inf.kernel_ms : 0.0;
         rep.line("  mem=%s waves=%d chunk=%d: %.0f props/ms, %lld/%d jobs finished, %.1f ms", mem == 1 ? "LDS" : "HBM",

@@ -78,6 +78,7 @@ int gpsat_solve_opts_default(gpsat_solve_opts* o)
     o->deadline_ms = 0.0;
     o->inst_conflicts = 0;
     o->job_order = 0;
+    o->telemetry = 0;
     return GPSAT_OK;
 }
 
@@ -174,6 +175,7 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     a.o.max_ticks = o.max_ms > 0 ? (int64_t)(o.max_ms * clock_khz) : 0;
     a.o.deadline_ticks = o.deadline_ms > 0 ? (int64_t)(o.deadline_ms * clock_khz) : 0;
     a.o.inst_conflicts = o.inst_conflicts;
+    a.o.telemetry = o.telemetry ? 1 : 0;
     a.max_n = d.max_n;
     a.wo_cap = max_L;
     a.wl_cap = 2 * d.max_n * a.o.watch_slack;
@@ -400,10 +402,12 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
         HIP_OR(hipMemcpy(model_rows, d_models, sizeof(uint32_t) * d.total_words, hipMemcpyDeviceToHost));
     std::vector<int64_t> js((size_t)std::max(n_jobs, 1) * JS_COUNT);
     HIP_OR(hipMemcpy(js.data(), d_jstats, sizeof(int64_t) * js.size(), hipMemcpyDeviceToHost));
-    for (int32_t j = 0; j < n_jobs; j++) {   // device clock ticks -> nanoseconds
-        int64_t& t = js[(size_t)j * JS_COUNT + JS_TICKS];
-        t = (int64_t)((double)t * 1e6 / (double)clock_khz);
-    }
+    static_assert(JS_COUNT == GPSAT_JOB_STATS, "job_stats stride of the C ABI");
+    for (int32_t j = 0; j < n_jobs; j++)   // device clock ticks -> nanoseconds
+        for (int f = JS_TICKS; f < JS_COUNT; f++) {
+            int64_t& t = js[(size_t)j * JS_COUNT + f];
+            t = (int64_t)((double)t * 1e6 / (double)clock_khz);
+        }
     if (job_stats) memcpy(job_stats, js.data(), sizeof(int64_t) * (size_t)n_jobs * JS_COUNT);
     if (var_bumps) HIP_OR(hipMemcpy(var_bumps, d_bumps, sizeof(int32_t) * var_base[N], hipMemcpyDeviceToHost));
 

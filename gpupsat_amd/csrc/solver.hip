@@ -99,6 +99,9 @@ struct Ctx {
     bool setup_ok;
     bool ran;            // the current job set up its instance (its per-var state is valid)
     int64_t conf_reported;  // conflicts of the current job already added to its instance pool
+    uint64_t ph_t;           // telemetry: tick at which the current phase began
+    int32_t ph_cur;          // telemetry: the current phase (Phase)
+    unsigned long long* ph_out;  // telemetry: this job's JS_PH_* counters (null: off)
     int64_t conflicts, decisions, props, learned, restarts;
     int32_t levelmax;
 };
@@ -694,6 +697,17 @@ __device__ void reset_instance(Ctx& J)
     J.larena_used = 0;
 }
 
+// Telemetry: charge the time since the last switch to the current phase and enter
+// `next`.  The counters are the job's JS_PH_* fields, added to without waiting.
+__device__ __forceinline__ void phase(Ctx& J, int32_t next)
+{
+    if (!J.ph_out) return;
+    const uint64_t now = wall_clock64();
+    if (lane_id() == 0) atomicAdd(&J.ph_out[J.ph_cur], (unsigned long long)(now - J.ph_t));
+    J.ph_t = now;
+    J.ph_cur = next;
+}
+
 // Adds the conflicts of this job not yet reported to its instance's pool; returns the
 // pool's new total (wave-uniform).
 __device__ int64_t report_conflicts(Ctx& J, const SolveArgs& a, int32_t inst)
@@ -718,6 +732,8 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
     if (o.inst_conflicts > 0 && (int64_t)__atomic_load_n(&a.inst_conf[jd.inst], __ATOMIC_RELAXED) >= o.inst_conflicts)
         return 2;
     const uint64_t t_start = wall_clock64();
+    J.ph_t = t_start;
+    J.ph_cur = PH_SETUP;
     if (J.setup_inst == jd.inst) {
         reset_instance(J);
     } else {
@@ -737,24 +753,29 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
     }
     __syncthreads();
     if (unsat) return 1;
+    phase(J, PH_BCP);
     if (propagate(J) >= 0) return 1;
 
     int32_t restart_thr = o.restart_first, restart_cnt = 0;
     const float decay_inv = 1.0f / o.var_decay;
     for (;;) {
+        phase(J, PH_BCP);
         const int32_t confl = propagate(J);
         if (confl >= 0) {
+            phase(J, PH_ANALYZE);
             J.conflicts++;
             restart_cnt++;
             if (J.dl == 0) return 1;
             int32_t bt, lbd;
             const int32_t nl = analyze(J, confl, bt, lbd);
             if (nl < 0) return 4;
+            phase(J, PH_BACKTRACK);
             if (nl == 1) {
                 backtrack(J, 0, o.decision_mode == 0);
                 assign(J, uni(J.lbuf[0]), -1);
             } else {
                 backtrack(J, bt, o.decision_mode == 0);
+                phase(J, PH_ANALYZE);
                 if (!(J.nlearnt < J.learnt_cap && J.larena_used + nl <= J.larena_cap)) {
                     reduce_db(J, 0);
                     if (!(J.nlearnt < J.learnt_cap && J.larena_used + nl <= J.larena_cap)) reduce_db(J, 2);
@@ -787,6 +808,7 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
             }
         } else {
             if (restart_cnt >= restart_thr) {
+                phase(J, PH_BACKTRACK);
                 restart_thr = (int32_t)((float)restart_thr * o.restart_factor);
                 restart_cnt = 0;
                 backtrack(J, 0, o.decision_mode == 0);
@@ -795,6 +817,7 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
                 if (J.nlearnt * 2 >= J.learnt_cap || J.larena_used * 2 >= J.larena_cap) reduce_db(J, 0);
                 continue;
             }
+            phase(J, PH_DECIDE);
             const int32_t v = pick_branch(J, o.decision_mode);
             if (v < 0) return 0;
             J.decisions++;
@@ -868,7 +891,12 @@ __global__ __launch_bounds__(64, 4) void k_solve(SolveArgs a)
         J.ran = false;
         J.conflicts = J.decisions = J.props = J.learned = J.restarts = 0;
         J.levelmax = 0;
+        J.ph_out = (a.o.telemetry && a.job_stats)
+                       ? (unsigned long long*)(a.job_stats + (int64_t)jd.order * JS_COUNT + JS_PH_SETUP) : nullptr;
+        J.ph_t = t_job;
+        J.ph_cur = PH_SETUP;
         const int32_t st = (deadline && t_job > deadline) ? 2 : run_job(J, a, jd, budget, deadline);
+        phase(J, PH_SETUP);   // charge the last phase
         const uint64_t job_ticks = wall_clock64() - t_job;
         __syncthreads();
         const int32_t inst = jd.inst;

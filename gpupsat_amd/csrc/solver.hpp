@@ -23,6 +23,7 @@ struct SolveOptsDev {
     int64_t max_ticks;        // per-job wall-clock budget in device clock ticks (<= 0: none) -> UNDEF
     int64_t deadline_ticks;   // whole-solve budget in ticks after the first wavefront starts (<= 0: none)
     int64_t inst_conflicts;   // conflict pool of each instance over all its jobs (<= 0: none)
+    int32_t telemetry;        // 1: accumulate per-phase ticks
 };
 
 struct JobDesc {
@@ -32,7 +33,12 @@ struct JobDesc {
     int32_t order;            // position of the job in the caller's job list
 };
 
-enum JobStat { JS_STATUS = 0, JS_CONFLICTS, JS_DECISIONS, JS_PROPS, JS_LEARNED, JS_RESTARTS, JS_LEVELMAX, JS_TICKS, JS_COUNT };
+enum JobStat {
+    JS_STATUS = 0, JS_CONFLICTS, JS_DECISIONS, JS_PROPS, JS_LEARNED, JS_RESTARTS, JS_LEVELMAX, JS_TICKS,
+    // per-phase ticks (telemetry): set-up, BCP, analysis + learning, backtrack, decide
+    JS_PH_SETUP, JS_PH_BCP, JS_PH_ANALYZE, JS_PH_BACKTRACK, JS_PH_DECIDE, JS_COUNT
+};
+enum Phase { PH_SETUP = 0, PH_BCP, PH_ANALYZE, PH_BACKTRACK, PH_DECIDE, PH_COUNT };
 
 struct SolveArgs {
     const InstMeta* meta;

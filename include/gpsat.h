@@ -40,6 +40,9 @@ extern "C" {
 #define GPSAT_UNSAT 1
 #define GPSAT_UNDEF 2
 
+/* int64 fields per job in gpsat_solve's job_stats */
+#define GPSAT_JOB_STATS 13
+
 typedef struct gpsat_batch gpsat_batch;
 
 /* Last error message of the calling thread ("" if none). */
@@ -191,6 +194,9 @@ typedef struct gpsat_solve_opts {
     int32_t job_order;       /* queue order: 0 largest instances first (default: the grid ends on
                                 short jobs), 1 smallest first (decides the most instances before a
                                 deadline), 2 as given; cubes of one instance stay adjacent */
+    int32_t telemetry;       /* 1: per-phase times in job_stats (set-up, BCP, analysis, backtrack,
+                                decide), the phases of RuntimeStatistics (src/Statistics/
+                                RuntimeStatistics.cu:246-360); 0: those fields stay 0 */
 } gpsat_solve_opts;
 
 typedef struct gpsat_solve_info {
@@ -211,8 +217,9 @@ int gpsat_solve_opts_default(gpsat_solve_opts *o);
  * initial VSIDS scores and phases, instances back to back (NULL: none).  Outputs:
  * status[n_inst] (0 SAT, 1 UNSAT, 2 UNDEF); model_rows (gpsat_batch_row_words(b, 1)
  * words: one literal bitmap per instance, valid where SAT); verified[n_inst] (1 where a
- * SAT model was re-checked on the GPU); job_stats[n_jobs*8] (status, conflicts,
- * decisions, propagations, learned, restarts, max level, job time in ns); var_bumps (per var, how
+ * SAT model was re-checked on the GPU); job_stats[n_jobs*GPSAT_JOB_STATS] (status, conflicts,
+ * decisions, propagations, learned, restarts, max level, job time in ns, then with telemetry the
+ * ns spent in set-up, BCP, conflict analysis + learning, backtracking, decisions); var_bumps (per var, how
  * often conflict analysis met it).  Every output may be NULL except status. */
 int gpsat_solve(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t *job_inst, const int32_t *cube_off,
                 const int32_t *cube_lits, int32_t n_jobs, const float *act_init, const int8_t *phase_init,

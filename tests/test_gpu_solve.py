@@ -232,3 +232,25 @@ def test_instance_conflict_pool_and_order(orc, order):
     ji = np.asarray(jobs[0])
     for i in range(3):
         assert js[ji == i, 1].sum() <= pool + 64 * int((ji == i).sum())
+
+
+def test_telemetry_phases(orc):
+    """Per-phase times (set-up, BCP, analysis, backtrack, decide): off by default, and when
+    on they add up to the job's own time and do not change a verdict."""
+    print("[solve] telemetry", flush=True)
+    b = W.random_3sat_batch(64, 60, 256, seed=77)
+    with G.GpuBatch(b) as gb:
+        r0 = gb.solve(opts())
+        r1 = gb.solve(opts(telemetry=1))
+    assert r0["status"].tolist() == r1["status"].tolist()
+    js0, js1 = r0["job_stats"], r1["job_stats"]
+    assert (js0[:, 8:] == 0).all()
+    ph = js1[:, 8:13]
+    assert (ph >= 0).all()
+    tot = ph.sum(axis=1)
+    t = js1[:, 7]
+    # the phases cover the job's time up to the clock reads around it
+    assert (tot <= t + 2000).all() and (tot >= 0.9 * t - 2000).all()
+    busy = js1[:, 3] > 0
+    assert (ph[busy, 1] > 0).all()                       # BCP time where there were propagations
+    assert (ph[js1[:, 1] > 0, 2] > 0).all()              # analysis time where there were conflicts
