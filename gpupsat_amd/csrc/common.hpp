@@ -34,14 +34,19 @@ __device__ __forceinline__ uint32_t bfi(uint32_t mask, uint32_t a, uint32_t b)
     return r;
 }
 
+// xor-swizzle inside each 32-lane group (ds_swizzle bitmask mode: and 0x1f, xor S).
+template <int S>
+__device__ __forceinline__ uint32_t swz_xor(uint32_t x)
+{
+    return (uint32_t)__builtin_amdgcn_ds_swizzle((int)x, 0x1F | (S << 10));
+}
+
 // 32x32 bit-matrix transpose held one row per lane in each 32-lane half: stage S swaps
 // the off-diagonal SxS blocks of every 2Sx2S block, between lanes i and i ^ S.  The
-// partner's row arrives through the VALU's cross-lane paths, not through LDS:
-//   S = 16  v_permlane16_swap (rows of 16 lanes trade places);
-//   S = 8   DPP row_ror:8 (lane i reads lane (i - 8) mod 16 = i ^ 8);
-//   S = 4   two DPP row_ror moves under bank masks (banks 0, 2 read i + 4, banks 1, 3 i - 4);
-//   S = 2, 1  DPP quad_perm [2,3,0,1] and [1,0,3,2].
-// For S = 16 and 8 the blocks are whole bytes and one v_perm_b32 with a per-lane byte
+// partner's row arrives through DPP where one move suffices -- S = 8 (row_ror:8: lane i
+// reads lane (i - 8) mod 16 = i ^ 8), S = 2 and S = 1 (quad_perm) -- and through the LDS
+// crossbar (ds_swizzle) for S = 16 and S = 4, where DPP would need extra moves.  For
+// S = 16 and 8 the blocks are whole bytes and one v_perm_b32 with a per-lane byte
 // selector merges own and partner bytes; for S = 4, 2, 1 the partner's row is rotated by
 // S one way or the other (a per-lane amount) and merged with a bit-field insert.
 struct Transposer {
@@ -50,8 +55,8 @@ struct Transposer {
     uint32_t keep[3];
     __device__ explicit Transposer(int lane)
     {
-        // v_perm_b32 byte selectors over {src0 (bytes 4-7), src1 (bytes 0-3)}
-        sel16 = (lane & 16) ? 0x07060302u : 0x05040100u;   // [y2 y3 x2 x3] | [x0 x1 y0 y1]
+        // v_perm_b32 byte selectors over {partner (bytes 4-7), own (bytes 0-3)}
+        sel16 = (lane & 16) ? 0x03020706u : 0x05040100u;   // [y2 y3 x2 x3] | [x0 x1 y0 y1]
         sel8 = (lane & 8) ? 0x03070105u : 0x06020400u;     // [y1 x1 y3 x3] | [x0 y0 x2 y2]
         const uint32_t H[3] = {0xF0F0F0F0u, 0xCCCCCCCCu, 0xAAAAAAAAu};
 #pragma unroll
@@ -71,18 +76,15 @@ struct Transposer {
     // afterwards lane r holds column r (bit c = old row c, bit r)
     __device__ __forceinline__ uint32_t operator()(uint32_t x) const
     {
-        // r[0]: rows 0 and 2 keep x, rows 1 and 3 receive the row below; r[1]: the reverse,
-        // so {r[1], r[0]} holds own and partner bytes in a lane-dependent order
-        const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
-        x = __builtin_amdgcn_perm(r[1], r[0], sel16);
-        uint32_t y = __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);      // row_ror:8
+        uint32_t y = swz_xor<16>(x);
+        x = __builtin_amdgcn_perm(y, x, sel16);
+        y = __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);      // row_ror:8
         x = __builtin_amdgcn_perm(y, x, sel8);
-        y = __builtin_amdgcn_update_dpp(0u, x, 0x12C, 0xF, 0x5, false);         // row_ror:12, banks 0, 2
-        y = __builtin_amdgcn_update_dpp(y, x, 0x124, 0xF, 0xA, false);          // row_ror:4, banks 1, 3
+        y = swz_xor<4>(x);
         x = bit_stage<0>(x, y);
-        y = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);                 // quad_perm [2,3,0,1]
+        y = __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);       // quad_perm [2,3,0,1]
         x = bit_stage<1>(x, y);
-        y = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);                 // quad_perm [1,0,3,2]
+        y = __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);       // quad_perm [1,0,3,2]
         return bit_stage<2>(x, y);
     }
 };

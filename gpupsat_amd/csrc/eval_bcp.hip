@@ -122,8 +122,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
     uint32_t* smem = lds_all + (size_t)(threadIdx.x / kWave) * lds_words;
     const Transposer tp(lane);
     const InstMeta mi = meta[inst];
-    const int W = words_per_row(mi.n);
-    const int m = mi.m;
+    const int W = __builtin_amdgcn_readfirstlane(words_per_row(mi.n));
+    const int m = __builtin_amdgcn_readfirstlane(mi.m);
     uint32_t* lt32 = smem;                       // 64*W words (2 per literal slot)
     uint32_t* stage = smem + 64 * W;             // A*W words
     uint32_t* conf = stage + 64 * W;             // 4 spare words before the staged arena
@@ -279,7 +279,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
 // LDS holds the literal masks twice: slot l (16 bytes) = {T(l), T(l ^ 1)}, the true and
 // the false mask of literal l, so a clause literal costs one 16-byte read and no select.
 // A propagated literal l is OR-ed into T(l) of slot l and into the false half of slot
-// l ^ 1.  The first 64*W words double as the row staging area.  LDS: (128*W + 4) words.
+// l ^ 1.  The first 64*W words double as the row staging area; slot 32*W is all zero.
+// LDS: (128*W + 4) words.
 constexpr int kFastW = kFast3MaxWords;
 constexpr int kFastG = kFast3MaxGroups;
 
@@ -298,10 +299,15 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     if (inst >= n_inst) return;
     uint32_t* smem = lds_all + (size_t)(threadIdx.x / kWave) * lds_words;
     const InstMeta mi = meta[inst];
-    const int W = words_per_row(mi.n);
-    const int m = mi.m;
+    // wave-uniform sizes in SGPRs: the group loops then branch on scalars
+    const int W = __builtin_amdgcn_readfirstlane(words_per_row(mi.n));
+    const int m = __builtin_amdgcn_readfirstlane(mi.m);
     const int G = (m + kWave - 1) / kWave;
-    uint32_t* dup = smem;                        // 32*W slots of 4 words
+    // slot 32*W stays zero: lanes past m evaluate a clause of three such literals, which
+    // has no status and no unit, so no group needs a validity mask
+    const uint32_t zslot = 32u * (uint32_t)W;
+    const uint32_t zpk = zslot | (zslot << 10) | (zslot << 20);
+    uint32_t* dup = smem;                        // 32*W slots of 4 words, then the zero slot
     const uint4* dup4 = reinterpret_cast<const uint4*>(dup);
     uint64_t* dup64 = reinterpret_cast<uint64_t*>(dup);
     const int64_t row0 = (int64_t)A * mi.word_base;
@@ -320,7 +326,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
 #pragma unroll
     for (int g = 0; g < kFastG; g++) {
         const int c = g * kWave + lane;
-        pk[g] = 0u;
+        pk[g] = zpk;
         if (c < m) {
             const int32_t* p = ilits + 3 * c;
             const uint32_t a = (uint32_t)__builtin_nontemporal_load(p);
@@ -341,6 +347,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     wave_sync();
     const Transposer tp(lane);
     const int half = lane >> 5, i = lane & 31;
+    if (lane < 4) dup[4 * zslot + lane] = 0u;
 #pragma unroll
     for (int j = 0; j < kFastW; j++) {
         if (j < W) {
@@ -356,7 +363,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     if (DO_EVAL) {
         uint32_t acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
         // software pipeline: the three 16-byte reads of group g+1 are issued before group
-        // g is evaluated; lanes past m read slot 0 (pk = 0) and are masked afterwards
+        // g is evaluated
         uint4 na = dup4[pk[0] & 1023u], nb = dup4[(pk[0] >> 10) & 1023u], nc = dup4[pk[0] >> 20];
 #pragma unroll
         for (int g = 0; g < kFastG; g++) {
@@ -368,11 +375,10 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                     nb = dup4[(p >> 10) & 1023u];
                     nc = dup4[p >> 20];
                 }
-                const uint64_t valid = (g * kWave + lane < m) ? ~0ull : 0ull;
                 const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
-                const uint64_t T = (u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y)) & valid;
-                const uint64_t all = Fa & Fb & Fc & valid;
-                const uint64_t unit = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T & valid;
+                const uint64_t T = (u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y));
+                const uint64_t all = Fa & Fb & Fc;
+                const uint64_t unit = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T;
                 const uint64_t unsat = all & ~T;
                 unsat_any |= unsat;
                 count_group(T | unit, unsat | unit, tp, acc);
@@ -431,7 +437,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                         nb = dup4[(pn >> 10) & 1023u];
                         nc = dup4[pn >> 20];
                     }
-                    const uint64_t valid = (g * kWave + lane < m) ? ~frozen : 0ull;
+                    const uint64_t valid = ~frozen;
                     const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
                     const uint64_t T = u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y);
                     const uint64_t all = Fa & Fb & Fc;
