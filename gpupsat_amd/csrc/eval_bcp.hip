@@ -116,10 +116,12 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
                                                    int32_t* __restrict__ passes_out, int n_inst, int lds_words)
 {
     extern __shared__ __align__(16) uint32_t lds_all[];
-    const int inst = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    // the wavefront's slot is uniform: its metadata then comes in with scalar loads
+    const int wslot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int inst = blockIdx.x * (blockDim.x / kWave) + wslot;
     const int lane = threadIdx.x % kWave;
     if (inst >= n_inst) return;
-    uint32_t* smem = lds_all + (size_t)(threadIdx.x / kWave) * lds_words;
+    uint32_t* smem = lds_all + (size_t)wslot * lds_words;
     const Transposer tp(lane);
     const InstMeta mi = meta[inst];
     const int W = __builtin_amdgcn_readfirstlane(words_per_row(mi.n));
@@ -281,6 +283,13 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
 // A propagated literal l is OR-ed into T(l) of slot l and into the false half of slot
 // l ^ 1.  The first 64*W words double as the row staging area; slot 32*W is all zero.
 // LDS: (128*W + 4) words.
+__device__ __forceinline__ int64_t uni64(int64_t x)
+{
+    const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
+    const uint32_t hi = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)((uint64_t)x >> 32));
+    return (int64_t)(((uint64_t)hi << 32) | lo);
+}
+
 constexpr int kFastW = kFast3MaxWords;
 constexpr int kFastG = kFast3MaxGroups;
 
@@ -294,14 +303,19 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                                                     int n_inst, int lds_words)
 {
     extern __shared__ __align__(16) uint32_t lds_all[];
-    const int inst = blockIdx.x * (blockDim.x / kWave) + (threadIdx.x / kWave);
+    // the wavefront's slot is uniform: its metadata then comes in with scalar loads
+    const int wslot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int inst = blockIdx.x * (blockDim.x / kWave) + wslot;
     const int lane = threadIdx.x % kWave;
     if (inst >= n_inst) return;
-    uint32_t* smem = lds_all + (size_t)(threadIdx.x / kWave) * lds_words;
+    uint32_t* smem = lds_all + (size_t)wslot * lds_words;
     const InstMeta mi = meta[inst];
-    // wave-uniform sizes in SGPRs: the group loops then branch on scalars
+    // the instance's metadata in SGPRs: the group loops branch on scalars, and the
+    // addresses of all loads are known before any of them is waited for
     const int W = __builtin_amdgcn_readfirstlane(words_per_row(mi.n));
     const int m = __builtin_amdgcn_readfirstlane(mi.m);
+    const int64_t lit_base = uni64(mi.lit_base);
+    const int64_t word_base = uni64(mi.word_base);
     const int G = (m + kWave - 1) / kWave;
     // slot 32*W stays zero: lanes past m evaluate a clause of three such literals, which
     // has no status and no unit, so no group needs a validity mask
@@ -310,29 +324,33 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     uint32_t* dup = smem;                        // 32*W slots of 4 words, then the zero slot
     const uint4* dup4 = reinterpret_cast<const uint4*>(dup);
     uint64_t* dup64 = reinterpret_cast<uint64_t*>(dup);
-    const int64_t row0 = (int64_t)A * mi.word_base;
-    const int32_t* ilits = lits + mi.lit_base;
+    const int64_t row0 = (int64_t)A * word_base;
+    const int32_t* ilits = lits + lit_base;
 
-    // rows: coalesced copy of the A*W words into LDS; literals: straight into VGPRs
+    // rows: coalesced copy of the A*W words into LDS; literals: straight into VGPRs.
+    // Every load sits in a scalar-guarded block with no use inside it (lanes past the end
+    // re-read the last word), so all of them are in flight together.
     const uint32_t* src = rows + row0;
     const int nw = A * W;
     uint32_t rbuf[kFastW];
 #pragma unroll
     for (int u = 0; u < kFastW; u++) {
-        const int w = u * kWave + lane;
-        rbuf[u] = (w < nw) ? __builtin_nontemporal_load(src + w) : 0u;
+        rbuf[u] = 0u;
+        if (u * kWave < nw) rbuf[u] = __builtin_nontemporal_load(src + min(u * kWave + lane, nw - 1));
     }
-    uint32_t pk[kFastG];
+    // literal loads: one scalar-guarded block per group with no use inside it, so all
+    // 3G loads are in flight together (lanes past m re-read clause m-1, which stays in
+    // bounds and is discarded when the triples are packed below)
+    uint32_t la[kFastG], lb[kFastG], lc[kFastG];
+    const int mc = m > 0 ? m - 1 : 0;
 #pragma unroll
     for (int g = 0; g < kFastG; g++) {
-        const int c = g * kWave + lane;
-        pk[g] = zpk;
-        if (c < m) {
-            const int32_t* p = ilits + 3 * c;
-            const uint32_t a = (uint32_t)__builtin_nontemporal_load(p);
-            const uint32_t b = (uint32_t)__builtin_nontemporal_load(p + 1);
-            const uint32_t d = (uint32_t)__builtin_nontemporal_load(p + 2);
-            pk[g] = a | (b << 10) | (d << 20);
+        la[g] = lb[g] = lc[g] = 0u;
+        if (g < G) {
+            const int32_t* p = ilits + 3 * min(g * kWave + lane, mc);
+            la[g] = (uint32_t)__builtin_nontemporal_load(p);
+            lb[g] = (uint32_t)__builtin_nontemporal_load(p + 1);
+            lc[g] = (uint32_t)__builtin_nontemporal_load(p + 2);
         }
     }
 #pragma unroll
@@ -357,6 +375,9 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
             dup[4 * (l ^ 1) + 2 + half] = x;
         }
     }
+    uint32_t pk[kFastG];
+#pragma unroll
+    for (int g = 0; g < kFastG; g++) pk[g] = (g * kWave + lane < m) ? (la[g] | (lb[g] << 10) | (lc[g] << 20)) : zpk;
     wave_sync();
 
     uint64_t unsat_any = 0;      // rows with a clause falsified by the input
