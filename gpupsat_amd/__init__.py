@@ -332,6 +332,24 @@ class GpuBatch:
         _check(lib().gpsat_eval_bcp_dev(self.h, rows.ptr, ctypes.c_int32(A), counts.ptr, out.ptr, conflict.ptr,
                                         props.ptr))
 
+    # ---- trajectory mode
+    def solve_ref(self, dead=None, max_iterations: int = 0, learned_cap: int = 0, learned_lits: int = 0) -> dict:
+        """The reference's own sequential search, one instance per lane (gpsat_solve_ref):
+        status (0 SAT, 1 UNSAT, 2 UNDEF, 4 capacity), model rows and per-instance stats
+        (decisions, conflicts, learned, iterations, restarts)."""
+        N = self.n_inst
+        status = np.zeros(N, dtype=np.int32)
+        rows = np.zeros(max(self.row_words(1), 1), dtype=np.uint32)
+        stats = np.zeros((max(N, 1), 5), dtype=np.int64)
+        d = None
+        if dead is not None:
+            d = np.ascontiguousarray(dead, dtype=np.uint8)
+            assert d.size == int(self.batch.n_vars.astype(np.int64).sum())
+        _check(lib().gpsat_solve_ref(self.h, _p(d) if d is not None else None, ctypes.c_int32(max_iterations),
+                                     ctypes.c_int32(learned_cap), ctypes.c_int32(learned_lits), _p(status), _p(rows),
+                                     _p(stats)))
+        return {"status": status, "model_rows": rows, "stats": stats[:N]}
+
     # ---- device preprocessing
     def preprocess(self):
         """The reference's preprocessing (duplicate/tautology removal, unit propagation)

@@ -10,7 +10,8 @@
 //   -l/--write-log                           --help  --version
 // The README's spellings --input, --threads, --blocks, --verbosity, --log are aliases.
 // Extra options of this engine: --max-conflicts N, --timeout-ms MS, --decision vsids|ref,
-// --learnt-cap N, --max-iterations N.
+// --learnt-cap N, --max-iterations N, --trajectory (a sequential run follows the reference's
+// own search step for step, MAX_ITERATIONS 1000 unless --max-iterations is given).
 #include <stdint.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -18,6 +19,7 @@
 #include <sys/stat.h>
 
 #include <algorithm>
+#include <chrono>
 #include <string>
 #include <vector>
 
@@ -32,6 +34,7 @@ struct Options {
     std::string strategy = "distributed";
     bool seq_as_parallel = false, write_log = false;
     long long max_conflicts = 0, max_iterations = 0;
+    bool trajectory = false;
     double timeout_ms = 0;
     int decision = 0, learnt_cap = 0;
     bool any = false;
@@ -56,6 +59,7 @@ void usage()
            "  --max-iterations arg                  Decision budget per job (0: none)\n"
            "  --timeout-ms arg                      Wall-clock budget per job (0: none)\n"
            "  --decision arg                        vsids (default) or ref (last free var, positive)\n"
+           "  --trajectory                          sequential run: the reference's own search, step for step\n"
            "  --learnt-cap arg                      Learned clauses held per job\n");
 }
 
@@ -110,6 +114,8 @@ bool parse_args(int argc, char** argv, Options& o)
             o.max_iterations = atoll(value(i).c_str());
         } else if (a == "--timeout-ms") {
             o.timeout_ms = atof(value(i).c_str());
+        } else if (a == "--trajectory") {
+            o.trajectory = true;
         } else if (a == "--decision") {
             const std::string d = value(i);
             o.decision = d == "ref" ? 1 : 0;
@@ -221,13 +227,13 @@ int main(int argc, char** argv)
             printf("Number of threads: %d\n", o.threads);
             printf("Job creation strategy: %s\n", o.strategy == "uniform" ? "uniform" : "distribution per thread");
         }
-        printf("Conflict analysis: ON (1-UIP)\n");
+        printf("Conflict analysis: ON (%s)\n", sequential && o.trajectory ? "decision scheme" : "1-UIP");
         printf("Unary clauses pre-processing is ON\n");
         printf("Conflict analysis is two wached literals\n");
     }
-    printf("VSIDS is %s\n", so.decision == 0 ? "ON" : "OFF");
+    printf("VSIDS is %s\n", so.decision == 0 && !(sequential && o.trajectory) ? "ON" : "OFF");
     printf("Restart is ON\n");
-    printf("Clause learning is ON with learnt clause capacity of %d\n", so.learnt_cap);
+    printf("Clause learning is ON with learnt clause capacity of %d\n", sequential && o.trajectory ? 20 : so.learnt_cap);
     printf("Simple jobs generation is OFF\n");
 
     std::vector<int8_t> value(std::max(n, 1), -1);   // -1 unassigned, 0 false, 1 true
@@ -266,11 +272,31 @@ int main(int argc, char** argv)
         std::vector<uint32_t> rows((2 * (size_t)n + 31) / 32 + 1);
         std::vector<int64_t> js((size_t)std::max(n_jobs, 1) * GPSAT_JOB_STATS);
         gpsat_solve_info info;
-        check(gpsat_solve(b, &so, sequential ? nullptr : ji.data(), sequential ? nullptr : co.data(),
-                          sequential ? nullptr : cl.data(), sequential ? 0 : n_jobs, nullptr, nullptr, &st,
-                          rows.data(), &verified, js.data(), nullptr, &info));
+        if (sequential && o.trajectory) {
+            // the reference's sequential search (SATSolver.cu:67-218), MAX_ITERATIONS 1000
+            std::vector<uint8_t> deadv(std::max(n, 1), 0);
+            for (int32_t v : dead) deadv[v] = 1;
+            int64_t stats[5];
+            const auto t0 = std::chrono::steady_clock::now();
+            check(gpsat_solve_ref(b, deadv.data(), o.max_iterations > 0 ? (int32_t)o.max_iterations : 1000, 0, 0, &st,
+                                  rows.data(), stats));
+            gpu_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (st == 4) {
+                fprintf(stderr, "trajectory mode ran out of learned-clause capacity\n");
+                gpsat_batch_destroy(b);
+                return 1;
+            }
+            if (o.verbosity >= 1)
+                printf("Decisions %lld, conflicts %lld, learned %lld, iterations %lld, restarts %lld\n",
+                       (long long)stats[0], (long long)stats[1], (long long)stats[2], (long long)stats[3],
+                       (long long)stats[4]);
+        } else {
+            check(gpsat_solve(b, &so, sequential ? nullptr : ji.data(), sequential ? nullptr : co.data(),
+                              sequential ? nullptr : cl.data(), sequential ? 0 : n_jobs, nullptr, nullptr, &st,
+                              rows.data(), &verified, js.data(), nullptr, &info));
+            gpu_ms = info.kernel_ms + info.verify_ms;
+        }
         gpsat_batch_destroy(b);
-        gpu_ms = info.kernel_ms + info.verify_ms;
         if (!sequential) {
             int solved_jobs = 0;
             for (int32_t j = 0; j < n_jobs; j++)

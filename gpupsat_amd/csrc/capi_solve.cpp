@@ -48,6 +48,7 @@ int hip_fail(const char* what, hipError_t e)
         if (_e != hipSuccess) return hip_fail(#expr, _e); \
     } while (0)
 
+
 extern "C" {
 
 int gpsat_solve_opts_default(gpsat_solve_opts* o)
@@ -447,3 +448,79 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
 }
 
 }  // extern "C"
+
+
+extern "C" int gpsat_solve_ref(gpsat_batch* b, const uint8_t* dead, int32_t max_iterations, int32_t learned_cap,
+                               int32_t learned_lits, int32_t* status, uint32_t* model_rows, int64_t* stats)
+{
+    if (!b || !status) {
+        set_error("null batch or status");
+        return GPSAT_ERR_ARG;
+    }
+    if (learned_cap < 0 || learned_lits < 0) {
+        set_error("negative capacity");
+        return GPSAT_ERR_ARG;
+    }
+    if (learned_cap == 0) learned_cap = 4096;
+    if (learned_lits == 0) learned_lits = 65536;
+    DeviceBatch& d = b->d;
+    const int32_t N = d.n_inst;
+    if (N == 0) return GPSAT_OK;
+    std::vector<RefMeta> rm(N);
+    std::vector<int64_t> var_base(N + 1, 0);
+    int64_t ws = 0;
+    for (int32_t i = 0; i < N; i++) {
+        const InstMeta& mi = d.h_meta[i];
+        const int32_t L = d.h_coff[mi.coff_base + mi.m];
+        if ((int64_t)mi.m + learned_cap > INT32_MAX / 2 || (int64_t)L + learned_lits > INT32_MAX / 2) {
+            set_error("instance too large for the trajectory mode");
+            return GPSAT_ERR_CAPACITY;
+        }
+        rm[i].ws = ws;
+        rm[i].ccap = mi.m + learned_cap;
+        rm[i].lcap = L + learned_lits;
+        rm[i].wtot = solve_ref_wtot(mi.n, L);
+        rm[i].pad = 0;
+        ws += (solve_ref_ws_ints(mi.n, mi.m, L, rm[i].ccap, rm[i].lcap) + 15) & ~(int64_t)15;
+        var_base[i + 1] = var_base[i] + mi.n;
+    }
+    hipStream_t st = d.stream;
+    DevMem mem;
+    RefMeta* d_rm;
+    int32_t *d_ws, *d_status;
+    int64_t *d_vb, *d_stats;
+    uint32_t* d_models;
+    uint8_t* d_dead = nullptr;
+    HIP_OR(mem.alloc(d_rm, N));
+    HIP_OR(mem.alloc(d_ws, (size_t)ws));
+    HIP_OR(mem.alloc(d_status, N));
+    HIP_OR(mem.alloc(d_vb, N + 1));
+    HIP_OR(mem.alloc(d_stats, (size_t)N * 5));
+    HIP_OR(mem.alloc(d_models, (size_t)std::max<int64_t>(d.total_words, 1)));
+    if (dead) {
+        HIP_OR(mem.alloc(d_dead, (size_t)std::max<int64_t>(var_base[N], 1)));
+        HIP_OR(hipMemcpyAsync(d_dead, dead, (size_t)var_base[N], hipMemcpyHostToDevice, st));
+    }
+    HIP_OR(hipMemcpyAsync(d_rm, rm.data(), sizeof(RefMeta) * N, hipMemcpyHostToDevice, st));
+    HIP_OR(hipMemcpyAsync(d_vb, var_base.data(), sizeof(int64_t) * (N + 1), hipMemcpyHostToDevice, st));
+    RefArgs a;
+    a.meta = d.meta;
+    a.lits = d.lits;
+    a.coff = d.coff;
+    a.rmeta = d_rm;
+    a.dead = d_dead;
+    a.var_base = d_vb;
+    a.ws = d_ws;
+    a.n_inst = N;
+    a.max_iterations = max_iterations;
+    a.status = d_status;
+    a.model_rows = d_models;
+    a.stats = d_stats;
+    HIP_OR(launch_solve_ref(a, st));
+    HIP_OR(hipMemcpyAsync(status, d_status, sizeof(int32_t) * N, hipMemcpyDeviceToHost, st));
+    if (model_rows)
+        HIP_OR(hipMemcpyAsync(model_rows, d_models, sizeof(uint32_t) * d.total_words, hipMemcpyDeviceToHost, st));
+    if (stats) HIP_OR(hipMemcpyAsync(stats, d_stats, sizeof(int64_t) * N * 5, hipMemcpyDeviceToHost, st));
+    HIP_OR(hipStreamSynchronize(st));
+    return GPSAT_OK;
+}

@@ -77,6 +77,34 @@ struct SolveArgs {
     SolveOptsDev o;
 };
 
+// Trajectory mode (solve_ref.hip): one instance per lane, the reference's own search.
+struct RefMeta {
+    int64_t ws;          // first int of the instance's workspace slice
+    int32_t ccap;        // clause slots (original + learned)
+    int32_t lcap;        // literal slots (original + learned)
+    int32_t wtot;        // watch-entry slots over all literals
+    int32_t pad;
+};
+
+struct RefArgs {
+    const InstMeta* meta;
+    const int32_t* lits;
+    const int32_t* coff;
+    const RefMeta* rmeta;
+    const uint8_t* dead;         // per var, instances back to back (null: none)
+    const int64_t* var_base;     // [n_inst]
+    int32_t* ws;
+    int32_t n_inst;
+    int32_t max_iterations;      // <= 0: none
+    int32_t* status;             // [n_inst]
+    uint32_t* model_rows;        // [total_words]
+    int64_t* stats;              // [n_inst * 5]: decisions, conflicts, learned, iterations, restarts
+};
+
+int64_t solve_ref_ws_ints(int32_t n, int32_t m, int32_t L, int32_t ccap, int32_t lcap);
+int32_t solve_ref_wtot(int32_t n, int32_t L);
+hipError_t launch_solve_ref(const RefArgs& a, hipStream_t st);
+
 size_t solve_state_lds_bytes(int32_t max_n);
 hipError_t launch_epoch_begin(const SolveArgs& a, hipStream_t st);
 hipError_t launch_epoch_end(const SolveArgs& a, int32_t* remaining, hipStream_t st);

@@ -56,3 +56,28 @@ def test_cli_random_instance_model(orc, tmp_path):
             model = np.array([1 if int(x) > 0 else 0 for x in lines[-2].split()[:-1]], dtype=np.int8)
             assert orc.verify_model(f, model)
     assert len((tmp_path / "autolog.txt").read_text().splitlines()) == b.n_inst
+
+
+def test_cli_trajectory_mode(orc, tmp_path):
+    """--trajectory: a sequential run follows the reference's own search; its counters and
+    verdict equal the oracle restatement's on the preprocessed formula, MAX_ITERATIONS 1000
+    included (n=100 instances mostly end UNDEFINED, like the reference)."""
+    print("[cli] trajectory mode", flush=True)
+    import re
+    fs = [W.random_3sat_batch(1, 50, 213, seed=600 + i).instance(0) for i in range(4)]
+    fs += [W.random_3sat_batch(1, 100, 426, seed=700 + i).instance(0) for i in range(3)]
+    for i, f in enumerate(fs):
+        p = tmp_path / f"t{i}.cnf"
+        p.write_text(f.to_dimacs())
+        lines = run("-i", str(p), "-b", "1", "-t", "1", "--trajectory")
+        pst, g, solved = orc.preprocess(f)
+        if pst != G.UNDEF:
+            continue
+        dead = np.zeros(f.n_vars, dtype=np.uint8)
+        dead[np.asarray(solved, dtype=np.int64) >> 1] = 1
+        st, _, stats = orc.solve_ref(g, dead=dead, max_iterations=1000)
+        assert verdict(lines) == ["SATISFIABLE", "UNSATISFIABLE", "UNDEFINED"][st]
+        m = [re.match(r"Decisions (\d+), conflicts (\d+), learned (\d+), iterations (\d+), restarts (\d+)", l)
+             for l in lines]
+        got = [list(map(int, x.groups())) for x in m if x][0]
+        assert got == stats.tolist(), (i, got, stats)
