@@ -119,6 +119,19 @@ def solve_leg(G, W, b, cpu):
         "props_per_s": float(js[:, 3].sum()) / (inf["kernel_ms"] * 1e-3),
         "waves": inf["waves"], "clause_mem": "LDS" if inf["clause_mem"] == 1 else "HBM",
     }
+    # the reference's own sequential search on the GPU (trajectory mode), same instances
+    gb.solve_ref(max_iterations=0, learned_cap=2048, learned_lits=16384)
+    t0 = time.perf_counter()
+    tr = gb.solve_ref(max_iterations=0, learned_cap=2048, learned_lits=16384)
+    t_ms = (time.perf_counter() - t0) * 1e3
+    tst = tr["status"]
+    leg["trajectory_mode"] = {
+        "what": "the reference's sequential search (last-free-var decisions, decision-scheme learning, "
+                "20-clause DB) on the GPU, one instance per lane, no iteration cap",
+        "instances_per_s": N / (t_ms * 1e-3), "wall_ms_incl_workspace_alloc": t_ms,
+        "decided": int((tst < 2).sum()), "verdicts_equal_cdcl": int(((tst == st) | (tst > 1) | (st > 1)).sum()),
+        "capacity_stops": int((tst == 4).sum()),
+    }
     # per-phase split of the device time (telemetry run on a 2,000-instance slice)
     sub = b.subset(range(min(2000, N)))
     with G.GpuBatch(sub) as gt:
