@@ -295,6 +295,11 @@ constexpr int kFastG = kFast3MaxGroups;
 
 __device__ __forceinline__ uint64_t u64(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
 
+// a native 4-word vector: 16-byte LDS reads of it stay single ds_read_b128 instructions
+// (the HIP uint4 struct is split into 8-byte halves by the compiler when its members are
+// consumed as 64-bit pairs)
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
 template <bool DO_EVAL, bool DO_BCP>
 __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ meta, const int32_t* __restrict__ lits,
                                                     const uint32_t* __restrict__ rows, int A, int32_t* __restrict__ counts,
@@ -322,7 +327,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     const uint32_t zslot = 32u * (uint32_t)W;
     const uint32_t zpk = zslot | (zslot << 10) | (zslot << 20);
     uint32_t* dup = smem;                        // 32*W slots of 4 words, then the zero slot
-    const uint4* dup4 = reinterpret_cast<const uint4*>(dup);
+    const u32x4* dup4 = reinterpret_cast<const u32x4*>(dup);
     uint64_t* dup64 = reinterpret_cast<uint64_t*>(dup);
     const int64_t row0 = (int64_t)A * word_base;
     const int32_t* ilits = lits + lit_base;
@@ -336,7 +341,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
 #pragma unroll
     for (int u = 0; u < kFastW; u++) {
         rbuf[u] = 0u;
-        if (u * kWave < nw) rbuf[u] = __builtin_nontemporal_load(src + min(u * kWave + lane, nw - 1));
+        if (u * kWave < nw) rbuf[u] = __builtin_nontemporal_load(src + u * kWave + min(lane, nw - 1 - u * kWave));
     }
     // literal loads: one scalar-guarded block per group with no use inside it, so all
     // 3G loads are in flight together (lanes past m re-read clause m-1, which stays in
@@ -347,21 +352,33 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     for (int g = 0; g < kFastG; g++) {
         la[g] = lb[g] = lc[g] = 0u;
         if (g < G) {
-            const int32_t* p = ilits + 3 * min(g * kWave + lane, mc);
+            const int32_t* p = ilits + 3 * g * kWave + 3 * min(lane, mc - g * kWave);
             la[g] = (uint32_t)__builtin_nontemporal_load(p);
             lb[g] = (uint32_t)__builtin_nontemporal_load(p + 1);
             lc[g] = (uint32_t)__builtin_nontemporal_load(p + 2);
         }
     }
+    // a conflicting row's output is its input: every output row starts as a copy (the
+    // rows that gain implications are rewritten at the end).  Word offsets are split
+    // into a scalar part and the lane, so no per-word lane index stays live.
+    uint32_t* dst = out_rows + row0;
 #pragma unroll
     for (int u = 0; u < kFastW; u++) {
-        const int w = u * kWave + lane;
-        if (w < nw) dup[w] = rbuf[u];
+        if (u * kWave < nw && lane < nw - u * kWave) {
+            dup[u * kWave + lane] = rbuf[u];
+            if (DO_BCP) dst[u * kWave + lane] = rbuf[u];
+        }
     }
     wave_sync();
+    // the rows are needed only to build the literal table; afterwards each lane keeps
+    // its row's literal count
     uint32_t in[kFastW];
+    int in_pop = 0;
 #pragma unroll
-    for (int j = 0; j < kFastW; j++) in[j] = (j < W && lane < A) ? dup[lane * W + j] : 0u;
+    for (int j = 0; j < kFastW; j++) {
+        in[j] = (j < W && lane < A) ? dup[lane * W + j] : 0u;
+        in_pop += __builtin_popcount(in[j]);
+    }
     wave_sync();
     const Transposer tp(lane);
     const int half = lane >> 5, i = lane & 31;
@@ -375,34 +392,47 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
             dup[4 * (l ^ 1) + 2 + half] = x;
         }
     }
+    // the packed triples are formed here, once: the empty asm keeps the compiler from
+    // sinking the packing into the groups, which would hold all 3G unpacked literals live
     uint32_t pk[kFastG];
 #pragma unroll
-    for (int g = 0; g < kFastG; g++) pk[g] = (g * kWave + lane < m) ? (la[g] | (lb[g] << 10) | (lc[g] << 20)) : zpk;
+    for (int g = 0; g < kFastG; g++) {
+        pk[g] = (lane < m - g * kWave) ? (la[g] | (lb[g] << 10) | (lc[g] << 20)) : zpk;
+        asm volatile("" : "+v"(pk[g]));
+    }
     wave_sync();
 
     uint64_t unsat_any = 0;      // rows with a clause falsified by the input
     if (DO_EVAL) {
         uint32_t acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
         // software pipeline: the three 16-byte reads of group g+1 are issued before group
-        // g is evaluated
-        uint4 na = dup4[pk[0] & 1023u], nb = dup4[(pk[0] >> 10) & 1023u], nc = dup4[pk[0] >> 20];
+        // g is evaluated.  Two named buffers alternate, so the unrolled groups need no
+        // register copies where a group's scalar guard joins.
+        const auto fetch = [&](uint32_t p, u32x4& a, u32x4& b, u32x4& c) {
+            a = dup4[p & 1023u];
+            b = dup4[(p >> 10) & 1023u];
+            c = dup4[p >> 20];
+        };
+        const auto eval_group = [&](const u32x4& qa, const u32x4& qb, const u32x4& qc) {
+            const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
+            const uint64_t T = (u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y));
+            const uint64_t all = Fa & Fb & Fc;
+            const uint64_t unit = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T;
+            const uint64_t unsat = all & ~T;
+            unsat_any |= unsat;
+            count_group(T | unit, unsat | unit, tp, acc);
+        };
+        u32x4 xa, xb, xc, ya, yb, yc;
+        fetch(pk[0], xa, xb, xc);
 #pragma unroll
-        for (int g = 0; g < kFastG; g++) {
+        for (int g = 0; g < kFastG; g += 2) {
             if (g < G) {
-                const uint4 qa = na, qb = nb, qc = nc;
-                if (g + 1 < kFastG) {
-                    const uint32_t p = pk[g + 1];
-                    na = dup4[p & 1023u];
-                    nb = dup4[(p >> 10) & 1023u];
-                    nc = dup4[p >> 20];
-                }
-                const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
-                const uint64_t T = (u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y));
-                const uint64_t all = Fa & Fb & Fc;
-                const uint64_t unit = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T;
-                const uint64_t unsat = all & ~T;
-                unsat_any |= unsat;
-                count_group(T | unit, unsat | unit, tp, acc);
+                if (g + 1 < kFastG) fetch(pk[g + 1], ya, yb, yc);
+                eval_group(xa, xb, xc);
+            }
+            if (g + 1 < G) {
+                if (g + 2 < kFastG) fetch(pk[g + 2], xa, xb, xc);
+                eval_group(ya, yb, yc);
             }
         }
         uint32_t tot[3];
@@ -427,37 +457,219 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
         {
             uint64_t cm = unsat_any;
             for (int v = lane; v < 16 * W; v += kWave) {
-                const uint4 q = dup4[2 * v];
+                const u32x4 q = dup4[2 * v];
                 cm |= u64(q.x, q.y) & u64(q.z, q.w);
             }
             frozen0 = wave_or64(cm);
         }
         uint64_t frozen = frozen0;
         int pass = 0;
-        bool changed = (frozen & amask) != amask;
-        while (changed) {
-            pass++;
-            bool my_change = false;
-            // the reads of group g+1 are issued before group g applies its implications,
-            // so they may miss them until the next pass (the fixpoint is unchanged: the
-            // loop ends only after a pass without a single write, or with every row in
-            // conflict).  The empty asm keeps the unpacked literals inside the pass
-            // instead of hoisted into 48 live registers.
-            uint32_t pn = pk[0];
-            asm volatile("" : "+v"(pn));
-            uint4 na = dup4[pn & 1023u], nb = dup4[(pn >> 10) & 1023u], nc = dup4[pn >> 20];
+        bool my_conflict;
+        int cnt = 0;            // literals in the closure of an open row
+        const uint64_t act = amask & ~frozen;
+        const int n_act = __builtin_popcountll(act);
+        if (n_act > 0 && n_act <= 32) {
+            // Few rows still open (the usual case after the evaluation already froze every
+            // row with a falsified clause): their bits are gathered into 32-bit masks, so
+            // the passes read 8 bytes per literal and run 32-bit logic.  Rows are
+            // independent, so the closure of each row is unchanged.
+            const bool me_act = (act >> lane) & 1ull;
+            const int k_me = __builtin_popcountll(act & ((1ull << lane) - 1ull));
+            uint32_t* ct = dup + 64 * W;            // slot l = {T(l), T(l ^ 1)}, then the zero slot
+            const uint2* ct2 = reinterpret_cast<const uint2*>(ct);
+            int a_k = 0;                            // lane k < n_act: row index of open row k
+            if (n_act <= 8) {
+                // the open rows' bits in the 64-bit table are still the input's (nothing has
+                // written it since it was built): compress each slot to the open rows
+                int al[8];
 #pragma unroll
-            for (int g = 0; g < kFastG; g++) {
-                if (g < G) {
-                    const uint32_t p = pn;
-                    const uint4 qa = na, qb = nb, qc = nc;
-                    if (g + 1 < kFastG) {
-                        pn = pk[g + 1];
-                        asm volatile("" : "+v"(pn));
-                        na = dup4[pn & 1023u];
-                        nb = dup4[(pn >> 10) & 1023u];
-                        nc = dup4[pn >> 20];
+                for (int k = 0; k < 8; k++) al[k] = 0;
+                {
+                    uint64_t r = act;
+#pragma unroll
+                    for (int k = 0; k < 8; k++) {
+                        if (k < n_act) {
+                            al[k] = __builtin_ctzll(r);
+                            r &= r - 1ull;
+                            if (lane == k) a_k = al[k];
+                        }
                     }
+                }
+                uint32_t ctt[8], ctf[8];
+#pragma unroll
+                for (int it = 0; it < 8; it++) {
+                    ctt[it] = ctf[it] = 0u;
+                    if (it * kWave < 32 * W && lane < 32 * W - it * kWave) {
+                        const u32x4 q = dup4[it * kWave + lane];
+#pragma unroll
+                        for (int k = 0; k < 8; k++) {
+                            if (k < n_act) {
+                                const int a = al[k];
+                                const uint32_t tw = a < 32 ? q.x : q.y, fw = a < 32 ? q.z : q.w;
+                                ctt[it] |= ((tw >> (a & 31)) & 1u) << k;
+                                ctf[it] |= ((fw >> (a & 31)) & 1u) << k;
+                            }
+                        }
+                    }
+                }
+                wave_sync();
+#pragma unroll
+                for (int it = 0; it < 8; it++) {
+                    if (it * kWave < 32 * W && lane < 32 * W - it * kWave) {
+                        const int v = it * kWave + lane;
+                        ct[2 * v] = ctt[it];
+                        ct[2 * v + 1] = ctf[it];
+                    }
+                }
+                if (lane < 2) ct[64 * W + lane] = 0u;
+            } else {
+                // lane k < n_act takes open row k: its row index a_k through an LDS map (the
+                // 64-bit table is dead from here on), its words straight from the input
+                wave_sync();
+                if (me_act) dup[k_me] = (uint32_t)lane;
+                if (lane < 2) ct[64 * W + lane] = 0u;
+                wave_sync();
+                a_k = lane < n_act ? (int)dup[lane] : 0;
+                uint32_t x[kFastW];
+#pragma unroll
+                for (int j = 0; j < kFastW; j++) x[j] = (j < W && lane < n_act) ? src[a_k * W + j] : 0u;
+                // all transposes first, then all table writes: the reads and writes of
+                // the table do not interleave, so the transposes overlap
+#pragma unroll
+                for (int j = 0; j < kFastW; j++)
+                    if (j < W) x[j] = tp(x[j]);      // lane l < 32: literal 32j+l
+                if (lane < 32) {
+#pragma unroll
+                    for (int j = 0; j < kFastW; j++) {
+                        if (j < W) {
+                            const int l = 32 * j + lane;
+                            ct[2 * l] = x[j];
+                            ct[2 * (l ^ 1) + 1] = x[j];
+                        }
+                    }
+                }
+            }
+            wave_sync();
+            const uint32_t amask32 = n_act >= 32 ? ~0u : ((1u << n_act) - 1u);
+            uint32_t fz = 0u;
+            bool changed = true;
+            while (changed) {
+                pass++;
+                // per-pass copy of the group count that the compiler cannot hoist: the
+                // group guards are then re-tested with scalar compares instead of being
+                // kept (and spilled) as masks across the passes
+                int Gp = G;
+                asm volatile("" : "+s"(Gp));
+                uint32_t chg = 0;       // any implication written by this lane
+                // same two-buffer pipeline as the evaluation; the empty asm keeps each
+                // group's unpacked literals inside the pass
+                const auto fetch = [&](uint32_t& p, int g, uint2& a, uint2& b, uint2& c) {
+                    p = pk[g];
+                    asm volatile("" : "+v"(p));
+                    a = ct2[p & 1023u];
+                    b = ct2[(p >> 10) & 1023u];
+                    c = ct2[p >> 20];
+                };
+                // one group: implications of its unit clauses, new conflicts; true once
+                // every open row is in conflict
+                const auto group = [&](uint32_t p, const uint2& qa, const uint2& qb, const uint2& qc) {
+                    const uint32_t Fa = qa.y, Fb = qb.y, Fc = qc.y;
+                    const uint32_t T = qa.x | qb.x | qc.x;
+                    const uint32_t all = Fa & Fb & Fc;
+                    const uint32_t newconf = all & ~T & ~fz;
+                    const uint32_t U = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T & ~fz;
+                    if (U) {
+                        const uint32_t la = p & 1023u, lb = (p >> 10) & 1023u, lc = p >> 20;
+                        const uint32_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
+                        if (Ua) {
+                            atomicOr(&ct[2 * la], Ua);
+                            atomicOr(&ct[2 * (la ^ 1u) + 1], Ua);
+                        }
+                        if (Ub) {
+                            atomicOr(&ct[2 * lb], Ub);
+                            atomicOr(&ct[2 * (lb ^ 1u) + 1], Ub);
+                        }
+                        if (Uc) {
+                            atomicOr(&ct[2 * lc], Uc);
+                            atomicOr(&ct[2 * (lc ^ 1u) + 1], Uc);
+                        }
+                    }
+                    chg |= U;
+                    if (ballot(newconf != 0)) {
+                        fz |= wave_or32(newconf);
+                        return (fz & amask32) == amask32;
+                    }
+                    return false;
+                };
+                uint32_t px, py;
+                uint2 xa, xb, xc, ya, yb, yc;
+                fetch(px, 0, xa, xb, xc);
+#pragma unroll
+                for (int g = 0; g < kFastG; g += 2) {
+                    if (g < Gp) {
+                        if (g + 1 < kFastG) fetch(py, g + 1, ya, yb, yc);
+                        if (group(px, xa, xb, xc)) break;
+                    }
+                    if (g + 1 < Gp) {
+                        if (g + 2 < kFastG) fetch(px, g + 2, xa, xb, xc);
+                        if (group(py, ya, yb, yc)) break;
+                    }
+                }
+                changed = ballot(chg != 0) != 0ull && (fz & amask32) != amask32;
+                wave_sync();
+            }
+            uint32_t cm32 = fz;
+            if ((fz & amask32) != amask32) {
+                uint32_t c = 0;
+                for (int v = lane; v < 16 * W; v += kWave) {
+                    const uint2 q = ct2[2 * v];
+                    c |= q.x & q.y;
+                }
+                cm32 |= wave_or32(c);
+            }
+            my_conflict = ((frozen >> lane) & 1ull) || (me_act && ((cm32 >> k_me) & 1u));
+            if ((cm32 & amask32) != amask32) {
+                // lane k < n_act transposes open row k back and stages it at its row's place
+                uint32_t x[kFastW];
+                int cnt_k = 0;
+#pragma unroll
+                for (int j = 0; j < kFastW; j++) {
+                    x[j] = 0u;
+                    if (j < W) {
+                        x[j] = tp(lane < 32 ? ct[2 * (32 * j + lane)] : 0u);   // lane k: word j of open row k
+                        cnt_k += __builtin_popcount(x[j]);
+                    }
+                }
+                if (lane < n_act) {
+#pragma unroll
+                    for (int j = 0; j < kFastW; j++)
+                        if (j < W) dup[a_k * W + j] = x[j];
+                }
+                cnt = __shfl(cnt_k, k_me);
+            }
+        } else {
+            bool changed = (frozen & amask) != amask;
+            while (changed) {
+                pass++;
+                // per-pass copy of the group count that the compiler cannot hoist: the
+                // group guards are then re-tested with scalar compares instead of being
+                // kept (and spilled) as masks across the passes
+                int Gp = G;
+                asm volatile("" : "+s"(Gp));
+                uint32_t chg = 0;       // any implication written by this lane
+                // the reads of group g+1 are issued before group g applies its implications,
+                // so they may miss them until the next pass (the fixpoint is unchanged: the
+                // loop ends only after a pass without a single write, or with every row in
+                // conflict).  The empty asm keeps the unpacked literals inside the pass
+                // instead of hoisted into 48 live registers.
+                const auto fetch = [&](uint32_t& p, int g, u32x4& a, u32x4& b, u32x4& c) {
+                    p = pk[g];
+                    asm volatile("" : "+v"(p));
+                    a = dup4[p & 1023u];
+                    b = dup4[(p >> 10) & 1023u];
+                    c = dup4[p >> 20];
+                };
+                const auto group = [&](uint32_t p, const u32x4& qa, const u32x4& qb, const u32x4& qc) {
                     const uint64_t valid = ~frozen;
                     const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
                     const uint64_t T = u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y);
@@ -479,48 +691,84 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                             atomicOr((unsigned long long*)&dup64[2 * lc], (unsigned long long)Uc);
                             atomicOr((unsigned long long*)&dup64[2 * (lc ^ 1u) + 1], (unsigned long long)Uc);
                         }
-                        my_change = true;
                     }
+                    chg |= (uint32_t)U | (uint32_t)(U >> 32);
                     if (ballot(newconf != 0)) {
                         frozen |= wave_or64(newconf);
-                        if ((frozen & amask) == amask) break;
+                        return (frozen & amask) == amask;
+                    }
+                    return false;
+                };
+                uint32_t px, py;
+                u32x4 xa, xb, xc, ya, yb, yc;
+                fetch(px, 0, xa, xb, xc);
+#pragma unroll
+                for (int g = 0; g < kFastG; g += 2) {
+                    if (g < Gp) {
+                        if (g + 1 < kFastG) fetch(py, g + 1, ya, yb, yc);
+                        if (group(px, xa, xb, xc)) break;
+                    }
+                    if (g + 1 < Gp) {
+                        if (g + 2 < kFastG) fetch(px, g + 2, xa, xb, xc);
+                        if (group(py, ya, yb, yc)) break;
                     }
                 }
+                changed = ballot(chg != 0) != 0ull && (frozen & amask) != amask;
+                wave_sync();
             }
-            changed = ballot(my_change) != 0ull && (frozen & amask) != amask;
+            // complementary implications are conflicts too (nothing to look for once every
+            // row is in conflict)
+            uint64_t cmask = frozen;
+            if ((frozen & amask) != amask) {
+                uint64_t cm = 0;
+                for (int v = lane; v < 16 * W; v += kWave) {
+                    const u32x4 q = dup4[2 * v];
+                    cm |= u64(q.x, q.y) & u64(q.z, q.w);
+                }
+                cmask |= wave_or64(cm);
+            }
+            my_conflict = (cmask >> lane) & 1ull;
+            const bool all_conflict = (cmask & amask) == amask;
+            if (!all_conflict) {
+                uint32_t out[kFastW];
+#pragma unroll
+                for (int j = 0; j < kFastW; j++) {
+                    out[j] = 0u;
+                    if (j < W) {
+                        out[j] = tp(dup[4 * (32 * j + i) + half]);   // lane a: word j of row a
+                        cnt += __builtin_popcount(out[j]);
+                    }
+                }
+                wave_sync();
+                if (lane < A && !my_conflict) {
+#pragma unroll
+                    for (int j = 0; j < kFastW; j++)
+                        if (j < W) dup[lane * W + j] = out[j];
+                }
+            }
+        }
+        // the closure contains the input, so the implied literals are the difference of the
+        // counts; the rows that gained some are staged in LDS and written again
+        const bool open_row = lane < A && !my_conflict;
+        const int np = open_row ? cnt - in_pop : 0;
+        const uint64_t grown = ballot(np > 0);
+        if (grown) {
+            // the copies above were stored by other lanes: they complete before any rewrite
+            asm volatile("s_waitcnt vmcnt(0)" ::: "memory");
             wave_sync();
-        }
-        // complementary implications are conflicts too
-        uint64_t cm = 0;
-        for (int v = lane; v < 16 * W; v += kWave) {
-            const uint4 q = dup4[2 * v];
-            cm |= u64(q.x, q.y) & u64(q.z, q.w);
-        }
-        const uint64_t cmask = frozen | wave_or64(cm);
-        const bool my_conflict = (cmask >> lane) & 1ull;
-        const bool all_conflict = (cmask & amask) == amask;
-        uint32_t np = 0;
-        uint32_t out[kFastW];
+            // row of word w: w * ceil(2^16 / W) >> 16 is exact for w < 1024 and W <= 16.
+            // The lane index, word count and output pointer are made opaque here so that the
+            // compiler does not keep the initial copy's addresses and guards alive (in
+            // spilled scalar registers) through the whole kernel to reuse them.
+            const uint32_t inv = (65536u + (uint32_t)W - 1u) / (uint32_t)W;
+            int ln = lane, nw2 = nw;
+            uint32_t* dst2 = dst;
+            asm volatile("" : "+v"(ln), "+s"(nw2), "+s"(dst2));
 #pragma unroll
-        for (int j = 0; j < kFastW; j++) {
-            out[j] = in[j];
-            if (j < W && !all_conflict) {
-                const uint32_t x = tp(dup[4 * (32 * j + i) + half]);   // lane a: word j of row a
-                if (!my_conflict) out[j] = x;
+            for (int u = 0; u < kFastW; u++) {
+                const int w = u * kWave + ln;
+                if (u * kWave < nw2 && w < nw2 && ((grown >> (((uint32_t)w * inv) >> 16)) & 1ull)) dst2[w] = dup[w];
             }
-            np += __builtin_popcount(out[j] & ~in[j]);
-        }
-        // stage the closure rows in LDS and store them coalesced
-        wave_sync();
-#pragma unroll
-        for (int j = 0; j < kFastW; j++)
-            if (j < W && lane < A) dup[lane * W + j] = out[j];
-        wave_sync();
-        uint32_t* dst = out_rows + row0;
-#pragma unroll
-        for (int u = 0; u < kFastW; u++) {
-            const int w = u * kWave + lane;
-            if (w < nw) dst[w] = dup[w];
         }
         if (lane < A) {
             conflict[(int64_t)inst * A + lane] = my_conflict ? 1 : 0;
