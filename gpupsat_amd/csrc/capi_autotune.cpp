@@ -160,7 +160,7 @@ int gpsat_autotune_solve(gpsat_batch* b, const gpsat_solve_opts* base, const int
     double best_rate = -1.0;
     int32_t bm = 2, bw = 0, bc = 0;
     for (int32_t mem : {1, 2}) {
-        for (int32_t per_cu : {4, 8, 12, 16}) {
+        for (int32_t per_cu : {8, 16, 24, 32}) {
             const double r = trial(mem, per_cu * cus, 0);
             if (r > best_rate) {
                 best_rate = r;

@@ -202,10 +202,15 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     }
     int waves = o.waves;
     if (waves <= 0) {
-        const int per_cu = (int)std::max<size_t>(1, std::min<size_t>(16, lds_cu / std::max<size_t>(lds_bytes, 1)));
+        const int per_cu = (int)std::max<size_t>(
+            1, std::min<size_t>(4 * kSolveWavesPerEu, lds_cu / std::max<size_t>(lds_bytes, 1)));
         waves = prop.multiProcessorCount * per_cu;
     }
     waves = std::max(1, std::min(waves, n_jobs));
+    // whole workgroups (surplus wavefronts find the queue empty and exit)
+    const int wpb = solve_waves_per_block(lds_bytes);
+    waves = (waves + wpb - 1) / wpb * wpb;
+    a.lds_per_wave = (int64_t)lds_bytes;
     // chunks of consecutive jobs let cubes of one instance reuse its watch lists; a few
     // chunks per wavefront keep the queue balanced
     a.chunk = (int32_t)std::max<int64_t>(1, std::min<int64_t>(64, (int64_t)n_jobs / ((int64_t)waves * 4)));

@@ -62,6 +62,16 @@ __device__ __forceinline__ uint64_t wave_max_u64(uint64_t k)
 
 __device__ __forceinline__ int32_t uni(int32_t x) { return __builtin_amdgcn_readfirstlane(x); }
 
+// Orders this wavefront's memory accesses across its lanes (per-wave LDS state and, in HBM
+// mode, its slice of the global workspace): waits for its outstanding memory operations,
+// then keeps the compiler from moving accesses across.  The wavefronts of a workgroup are
+// independent, so no workgroup barrier is involved.
+__device__ __forceinline__ void wave_mem_sync()
+{
+    __builtin_amdgcn_s_waitcnt(0);
+    wave_sync();
+}
+
 struct Ctx {
     // instance (uniform)
     int32_t n, m, inst;
@@ -243,7 +253,7 @@ __device__ void backtrack(Ctx& J, int32_t b, bool save_phase)
         J.reason[v] = -1;
         if (save_phase) J.phase[v] = (int8_t)(u & 1);
     }
-    __syncthreads();
+    wave_mem_sync();
     J.tsize = start;
     J.qhead = start;
     J.dl = b;
@@ -304,7 +314,7 @@ __device__ int32_t analyze(Ctx& J, int32_t confl, int32_t& bt, int32_t& lbd)
         c = uni(J.reason[p >> 1]);
     }
     J.lbuf[0] = p ^ 1;
-    __syncthreads();
+    wave_mem_sync();
     // backjump level = highest level among the other literals; LBD = distinct levels
     J.stamp++;
     uint64_t best = 0;
@@ -325,13 +335,13 @@ __device__ int32_t analyze(Ctx& J, int32_t confl, int32_t& bt, int32_t& lbd)
         bt = uni((int32_t)(best >> 32));
         const int32_t bi = uni(0x7fffffff - (int32_t)(uint32_t)best);
         const int32_t t1 = J.lbuf[1], tb = J.lbuf[bi];
-        __syncthreads();
+        wave_mem_sync();
         J.lbuf[1] = tb;
         J.lbuf[bi] = t1;
     } else {
         bt = 0;
     }
-    __syncthreads();
+    wave_mem_sync();
     return nl;
 }
 
@@ -403,7 +413,7 @@ __device__ bool attach_level0(Ctx& J, int32_t i)
     } else {
         watch_learned(J, J.m + i, nf[0], nf[1]);
     }
-    __syncthreads();
+    wave_mem_sync();
     return ok;
 }
 
@@ -422,7 +432,7 @@ __device__ void reduce_db(Ctx& J, int mode)
         const int32_t r = J.reason[v];
         if (r >= m && J.level[v] > 0) atomicOr(&J.lmeta[4 * (r - m) + 2], F_LOCKED);   // level-0 reasons are never read
     }
-    __syncthreads();
+    wave_mem_sync();
     int32_t ns = 0;
     for (int32_t i0 = 0; i0 < J.nlearnt; i0 += kWave) {
         const int32_t i = i0 + lane;
@@ -440,7 +450,7 @@ __device__ void reduce_db(Ctx& J, int mode)
             keep = (fl & F_LOCKED) || (!del && wanted);
         }
         const uint64_t km = ballot(keep);
-        __syncthreads();
+        wave_mem_sync();
         const int32_t ni = ns + prefix_count(km);
         if (i < J.nlearnt) remap[i] = keep ? ni : -1;
         if (keep) {
@@ -452,7 +462,7 @@ __device__ void reduce_db(Ctx& J, int mode)
             J.wpair[2 * (m + ni) + 1] = w1;
         }
         ns += popc64(km);
-        __syncthreads();
+        wave_mem_sync();
     }
     // compact literals (forward copy; a destination never lies above its source)
     int32_t pos = 0;
@@ -462,9 +472,9 @@ __device__ void reduce_db(Ctx& J, int mode)
         for (int32_t k0 = 0; k0 < len; k0 += kWave) {
             const int32_t k = k0 + lane;
             const int32_t x = k < len ? J.larena[st + k] : 0;
-            __syncthreads();
+            wave_mem_sync();
             if (k < len) J.larena[pos + k] = x;
-            __syncthreads();
+            wave_mem_sync();
         }
         J.lmeta[4 * i] = pos;
         pos += len;
@@ -475,7 +485,7 @@ __device__ void reduce_db(Ctx& J, int mode)
         if (r >= m) J.reason[v] = remap[r - m] >= 0 ? m + remap[r - m] : -1;
     }
     for (int32_t l = lane; l < 2 * J.n; l += kWave) J.wcnt_l[l] = 0;
-    __syncthreads();
+    wave_mem_sync();
     for (int32_t i = lane; i < ns; i += kWave) {
         if (J.lmeta[4 * i + 2] & F_DELETED) continue;
         const int32_t c = m + i;
@@ -492,7 +502,7 @@ __device__ void reduce_db(Ctx& J, int mode)
     }
     J.nlearnt = ns;
     J.larena_used = pos;
-    __syncthreads();
+    wave_mem_sync();
 }
 
 __device__ int32_t pick_branch(Ctx& J, int mode)
@@ -518,7 +528,7 @@ __device__ void rescale_activity(Ctx& J)
 {
     for (int32_t v = lane_id(); v < J.n; v += kWave) J.act[v] *= 1e-30f;
     J.inc *= 1e-30f;
-    __syncthreads();
+    wave_mem_sync();
 }
 
 __device__ __forceinline__ int32_t load_status(int32_t* p)
@@ -610,11 +620,11 @@ __device__ bool setup_instance(Ctx& J, const SolveArgs& a, int32_t inst)
         J.wcnt_o[l] = 0;
         J.wcnt_l[l] = 0;
     }
-    __syncthreads();
+    wave_mem_sync();
     // occurrence counts give the capacity of every original-clause watch list
     const int32_t L = uni(J.coff[m]);
     for (int32_t k = lane; k < L; k += kWave) atomicAdd(&J.wcnt_o[J.lits[k]], 1);
-    __syncthreads();
+    wave_mem_sync();
     int32_t carry = 0;
     for (int32_t l0 = 0; l0 < n2; l0 += kWave) {
         const int32_t l = l0 + lane;
@@ -627,7 +637,7 @@ __device__ bool setup_instance(Ctx& J, const SolveArgs& a, int32_t inst)
         carry += uni(__shfl(inc, 63));
     }
     if (lane == 0) J.wstart[n2] = carry;
-    __syncthreads();
+    wave_mem_sync();
     // watch the first two literals of every clause; units and empty clauses at level 0
     bool unsat = false;
     for (int32_t c0 = 0; c0 < m && !unsat; c0 += kWave) {
@@ -670,7 +680,7 @@ __device__ bool setup_instance(Ctx& J, const SolveArgs& a, int32_t inst)
         }
     }
     J.base_trail = J.tsize;
-    __syncthreads();
+    wave_mem_sync();
     return !unsat;
 }
 
@@ -689,7 +699,7 @@ __device__ void reset_instance(Ctx& J)
     }
     for (int32_t l = lane; l < 2 * J.n; l += kWave) J.wcnt_l[l] = 0;
     for (int32_t v = lane; v < J.n; v += kWave) J.bumps[v] = 0;
-    __syncthreads();
+    wave_mem_sync();
     J.tsize = J.base_trail;
     J.qhead = 0;
     J.dl = 0;
@@ -751,7 +761,7 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
         if (val == -1) unsat = true;
         else if (val == 0) assign(J, u, -1);
     }
-    __syncthreads();
+    wave_mem_sync();
     if (unsat) return 1;
     phase(J, PH_BCP);
     if (propagate(J) >= 0) return 1;
@@ -782,7 +792,7 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
                 }
                 if (J.nlearnt < J.learnt_cap && J.larena_used + nl <= J.larena_cap) {
                     const int32_t cid = store_learned(J, nl, lbd);
-                    __syncthreads();
+                    wave_mem_sync();
                     watch_learned(J, cid, uni(J.lbuf[0]), uni(J.lbuf[1]));
                     assign(J, uni(J.lbuf[0]), cid);
                 } else {
@@ -792,11 +802,11 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
                     restart_cnt = 0;
                     reduce_db(J, 2);
                     const int32_t cid = store_learned(J, nl, lbd);
-                    __syncthreads();
+                    wave_mem_sync();
                     if (!attach_level0(J, cid - m)) return 1;
                 }
             }
-            __syncthreads();
+            wave_mem_sync();
             J.inc *= decay_inv;
             if (J.inc > 1e30f) rescale_activity(J);
             if (max_conflicts > 0 && J.conflicts >= max_conflicts) return 2;
@@ -840,14 +850,20 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
 }  // namespace
 
 template <bool kLds>
-__global__ __launch_bounds__(64, 4) void k_solve(SolveArgs a)
+__global__ __launch_bounds__(kSolveWavesPerBlock * kWave) __attribute__((amdgpu_waves_per_eu(kSolveWavesPerEu, kSolveWavesPerEu)))
+void k_solve(SolveArgs a)
 {
-    extern __shared__ __align__(16) char smem[];
+    extern __shared__ __align__(16) char smem_all[];
     const int lane = lane_id();
+    // the wavefronts of a workgroup are independent (separate jobs, private LDS regions,
+    // wave-level ordering only): several per workgroup let a CU hold more than 16
+    const int wslot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x / kWave) + wslot;
+    char* smem = smem_all + (size_t)wslot * a.lds_per_wave;
     Ctx J;
     carve_state(J, smem, a.max_n);
     int32_t* cpart = kLds ? (int32_t*)(smem + Layout::state_bytes(a.max_n))
-                          : a.ws + (int64_t)blockIdx.x * a.ws_ints_per_wave;
+                          : a.ws + gwave * a.ws_ints_per_wave;
     carve_clauses(J, cpart, a);
     J.setup_inst = -1;
     J.setup_ok = false;
@@ -898,7 +914,7 @@ __global__ __launch_bounds__(64, 4) void k_solve(SolveArgs a)
         const int32_t st = (deadline && t_job > deadline) ? 2 : run_job(J, a, jd, budget, deadline);
         phase(J, PH_SETUP);   // charge the last phase
         const uint64_t job_ticks = wall_clock64() - t_job;
-        __syncthreads();
+        wave_mem_sync();
         const int32_t inst = jd.inst;
         if (a.o.inst_conflicts > 0 && J.ran) (void)report_conflicts(J, a, inst);
         if (st == 0) {
@@ -952,7 +968,7 @@ __global__ __launch_bounds__(64, 4) void k_solve(SolveArgs a)
             s[JS_LEVELMAX] = max(s[JS_LEVELMAX], (int64_t)J.levelmax);
             s[JS_TICKS] += (int64_t)job_ticks;
         }
-        __syncthreads();
+        wave_mem_sync();
     }
 }
 
@@ -1008,11 +1024,22 @@ size_t solve_state_lds_bytes(int32_t max_n) { return Layout::state_bytes(max_n);
 
 hipError_t launch_solve(const SolveArgs& a, bool lds_clauses, int waves, size_t lds_bytes, hipStream_t st)
 {
+    // waves is a multiple of the wavefronts per workgroup (solve_waves_per_block)
+    const int wpb = solve_waves_per_block(lds_bytes);
+    if (waves % wpb != 0 || (size_t)a.lds_per_wave != lds_bytes) return hipErrorInvalidValue;
+    const dim3 grid(waves / wpb), block(kWave * wpb);
     if (lds_clauses)
-        hipLaunchKernelGGL(k_solve<true>, dim3(waves), dim3(kWave), lds_bytes, st, a);
+        hipLaunchKernelGGL(k_solve<true>, grid, block, lds_bytes * wpb, st, a);
     else
-        hipLaunchKernelGGL(k_solve<false>, dim3(waves), dim3(kWave), lds_bytes, st, a);
+        hipLaunchKernelGGL(k_solve<false>, grid, block, lds_bytes * wpb, st, a);
     return hipGetLastError();
+}
+
+int solve_waves_per_block(size_t lds_bytes)
+{
+    int wpb = kSolveWavesPerBlock;
+    while (wpb > 1 && (size_t)wpb * lds_bytes > 64 * 1024) wpb >>= 1;
+    return wpb;
 }
 
 }  // namespace gpsat

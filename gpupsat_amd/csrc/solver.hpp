@@ -9,6 +9,11 @@ namespace gpsat {
 
 // Solver options (the strategy surfaces of the reference's compile-time Configs.cuh,
 // chosen at run time here).
+// k_solve launch shape: independent wavefronts per workgroup, and the wavefronts per SIMD
+// the register allocation is held to (8 -> at most 64 VGPRs per lane)
+constexpr int kSolveWavesPerBlock = 4;
+constexpr int kSolveWavesPerEu = 8;
+
 struct SolveOptsDev {
     int32_t decision_mode;    // 0: VSIDS + phase saving; 1: reference scheme (last free var, positive)
     int32_t restart_first;    // conflicts before the first restart (GEOMETRIC_CONFLICTS_BEFORE_RESTART)
@@ -71,6 +76,7 @@ struct SolveArgs {
     int8_t* phase_store;
     int32_t* ws;                    // global workspace
     int64_t ws_ints_per_wave;
+    int64_t lds_per_wave;           // bytes of each wavefront's LDS region
     // per-wave clause-part layout (ints), identical in LDS and global mode
     int32_t wo_cap, wl_cap, wpair_cap, lmeta_cap;
     int32_t max_n;
@@ -109,5 +115,6 @@ size_t solve_state_lds_bytes(int32_t max_n);
 hipError_t launch_epoch_begin(const SolveArgs& a, hipStream_t st);
 hipError_t launch_epoch_end(const SolveArgs& a, int32_t* remaining, hipStream_t st);
 hipError_t launch_solve(const SolveArgs& a, bool lds_clauses, int waves, size_t lds_bytes, hipStream_t st);
+int solve_waves_per_block(size_t lds_bytes);
 
 }  // namespace gpsat
