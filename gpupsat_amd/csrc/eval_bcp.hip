@@ -276,12 +276,15 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
 // with n <= 256 (W <= 16 words per row) and m <= 1024 (<= 16 clause groups).  Every
 // global load of the instance (its A rows and its 3m literals) is issued before the
 // first one is used; each lane then keeps its 16 clauses as packed 10-bit literal
-// triples in VGPRs and its input row in VGPRs, so the BCP passes touch LDS only.
+// triples in VGPRs, so the BCP passes touch LDS only.  With BCP, the input rows are copied
+// to the output as they arrive; only rows that gain implications are written again.
 //
 // LDS holds the literal masks twice: slot l (16 bytes) = {T(l), T(l ^ 1)}, the true and
 // the false mask of literal l, so a clause literal costs one 16-byte read and no select.
 // A propagated literal l is OR-ed into T(l) of slot l and into the false half of slot
 // l ^ 1.  The first 64*W words double as the row staging area; slot 32*W is all zero.
+// With at most 32 open rows, BCP runs on a compacted table of 8-byte slots
+// {T(l), T(l ^ 1)} over the open rows, at words [64*W, 128*W + 2).
 // LDS: (128*W + 4) words.
 __device__ __forceinline__ int64_t uni64(int64_t x)
 {
