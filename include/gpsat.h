@@ -175,7 +175,8 @@ typedef struct gpsat_solve_opts {
     float var_decay;         /* VSIDS decay per conflict */
     int32_t phase_init;      /* initial phase without a hint: 0 false, 1 true */
     double max_ms;           /* per-job wall-clock budget in ms (<= 0: none), exceeded -> UNDEF */
-    int32_t waves;           /* persistent wavefronts (0: automatic) */
+    int32_t waves;           /* persistent wavefronts, rounded up to whole workgroups of 4
+                                (0: automatic, up to 32 per CU as LDS allows) */
     int32_t clause_mem;      /* clause part of the job state: 0 automatic, 1 LDS, 2 HBM */
     int32_t verify;          /* 1: re-check every SAT model with the clause-evaluation kernel */
     int32_t chunk;           /* jobs per queue grab (0: automatic); cubes of one instance in one
@@ -245,7 +246,7 @@ int gpsat_solve_ref(gpsat_batch *b, const uint8_t *dead, int32_t max_iterations,
  * fixes its strategies at compile time, src/SATSolver/Configs.cuh).
  * gpsat_autotune_solve: short trial solves of a sample of the jobs (whole instances, at
  * most sample_jobs jobs; 0: 4096) under each launch shape -- clause part in LDS or HBM,
- * 4/8/12/16 persistent wavefronts per CU, then 1/8/32/64 jobs per queue grab when there
+ * 8/16/24/32 persistent wavefronts per CU, then 1/8/32/64 jobs per queue grab when there
  * are cubes -- each with deadline trial_ms (0: 300).  *best = *base with waves,
  * clause_mem and chunk of the shape with the most propagations per ms of kernel time.
  * Verdicts do not depend on the shape.  report (optional) receives one line per trial.
