@@ -248,7 +248,8 @@ class SolveOpts(ctypes.Structure):
         ("clause_mem", ctypes.c_int32), ("verify", ctypes.c_int32), ("chunk", ctypes.c_int32),
         ("epoch_conflicts", ctypes.c_int64), ("epoch_growth", ctypes.c_float), ("max_epochs", ctypes.c_int32),
         ("deadline_ms", ctypes.c_double), ("inst_conflicts", ctypes.c_int64), ("job_order", ctypes.c_int32),
-        ("telemetry", ctypes.c_int32),
+        ("telemetry", ctypes.c_int32), ("learned_out", ctypes.POINTER(ctypes.c_int32)),
+        ("learned_cap", ctypes.c_int64),
     ]
 
     @staticmethod
@@ -262,13 +263,29 @@ class SolveOpts(ctypes.Structure):
         return o
 
 
+def parse_learned(d: np.ndarray) -> list:
+    """Split a learned-clause dump into (instance, [clause literal lists]) blocks."""
+    out, p = [], 0
+    while p < len(d):
+        inst, n = int(d[p]), int(d[p + 1])
+        body = d[p + 2:p + 2 + n]
+        ends = np.flatnonzero(body == -1)
+        clauses, s = [], 0
+        for e in ends:
+            clauses.append(body[s:e].tolist())
+            s = e + 1
+        out.append((inst, clauses))
+        p += 2 + n
+    return out
+
+
 class SolveInfo(ctypes.Structure):
     _fields_ = [
         ("kernel_ms", ctypes.c_double), ("verify_ms", ctypes.c_double), ("waves", ctypes.c_int32),
         ("clause_mem", ctypes.c_int32), ("lds_bytes", ctypes.c_int64), ("workspace_bytes", ctypes.c_int64),
         ("n_sat", ctypes.c_int32), ("n_unsat", ctypes.c_int32), ("n_undef", ctypes.c_int32),
         ("n_verified", ctypes.c_int32), ("n_errors", ctypes.c_int32), ("epochs", ctypes.c_int32),
-        ("chunk", ctypes.c_int32),
+        ("chunk", ctypes.c_int32), ("learned_len", ctypes.c_int64), ("learned_need", ctypes.c_int64),
     ]
 
     def as_dict(self):
@@ -403,9 +420,17 @@ class GpuBatch:
 
     # ---- solve path
     def solve(self, opts: Optional[SolveOpts] = None, jobs=None, act_init=None, phase_init=None,
-              want_models: bool = True, want_bumps: bool = False) -> dict:
-        """jobs: None (one job per instance) or (job_inst int32[J], cube_off int32[J+1], cube_lits int32[])."""
-        o = opts if opts is not None else SolveOpts.default()
+              want_models: bool = True, want_bumps: bool = False, learned_cap: int = 0) -> dict:
+        """jobs: None (one job per instance) or (job_inst int32[J], cube_off int32[J+1], cube_lits int32[]).
+        learned_cap > 0: also return "learned", a list of (instance, clauses) blocks, one per job
+        that ran (its held learned clauses and level-0 literals, each implied by the formula and
+        the job's cube), as many as fit in learned_cap int32 slots."""
+        o = SolveOpts.from_buffer_copy(opts) if opts is not None else SolveOpts.default()
+        dump = None
+        if learned_cap > 0:
+            dump = np.zeros(int(learned_cap), dtype=np.int32)
+            o.learned_out = dump.ctypes.data_as(ctypes.POINTER(ctypes.c_int32))
+            o.learned_cap = int(learned_cap)
         N = self.n_inst
         status = np.full(N, 2, dtype=np.int32)
         verified = np.zeros(N, dtype=np.int32)
@@ -433,6 +458,8 @@ class GpuBatch:
             out["model_rows"] = rows
         if want_bumps:
             out["var_bumps"] = bumps[:nv]
+        if dump is not None:
+            out["learned"] = parse_learned(dump[:info.learned_len])
         return out
 
     def models(self, model_rows: np.ndarray) -> list:

@@ -80,6 +80,8 @@ int gpsat_solve_opts_default(gpsat_solve_opts* o)
     o->inst_conflicts = 0;
     o->job_order = 0;
     o->telemetry = 0;
+    o->learned_out = nullptr;
+    o->learned_cap = 0;
     return GPSAT_OK;
 }
 
@@ -284,6 +286,17 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     a.job_stats = d_jstats;
     a.var_bumps = d_bumps;
     a.ws = d_ws;
+    if (o.learned_out && o.learned_cap > 0) {
+        if (o.learned_cap > ((int64_t)1 << 31)) {
+            set_error("learned_cap above 2^31 slots");
+            return GPSAT_ERR_ARG;
+        }
+        HIP_OR(mem.alloc(a.learned_out, (size_t)o.learned_cap));
+        HIP_OR(mem.alloc(a.learned_pos, 2));
+        HIP_OR(hipMemsetAsync(a.learned_pos, 0, 2 * sizeof(unsigned long long), st));
+        a.learned_need = a.learned_pos + 1;
+        a.learned_cap = o.learned_cap;
+    }
 
     const bool epochs = o.epoch_conflicts > 0;
     int32_t epochs_run = 1;
@@ -426,6 +439,13 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     inf.workspace_bytes = use_lds ? 0 : (int64_t)waves * a.ws_ints_per_wave * 4;
     inf.epochs = epochs_run;
     inf.chunk = a.chunk;
+    if (a.learned_out) {
+        unsigned long long pos[2];
+        HIP_OR(hipMemcpy(pos, a.learned_pos, sizeof(pos), hipMemcpyDeviceToHost));
+        inf.learned_len = (int64_t)pos[0];
+        inf.learned_need = (int64_t)pos[1];
+        if (pos[0]) HIP_OR(hipMemcpy(o.learned_out, a.learned_out, sizeof(int32_t) * pos[0], hipMemcpyDeviceToHost));
+    }
     for (int32_t j = 0; j < n_jobs; j++)
         if (js[(size_t)j * JS_COUNT + JS_STATUS] == 4) inf.n_errors++;
     for (int32_t i = 0; i < N; i++) {

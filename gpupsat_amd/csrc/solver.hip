@@ -729,6 +729,51 @@ __device__ int64_t report_conflicts(Ctx& J, const SolveArgs& a, int32_t inst)
     return (int64_t)__shfl(tot, 0);
 }
 
+// Testing aid: append the job's held learned clauses and its level-0 literals (as unit
+// clauses) to the dump as one block [inst, slots, then per clause its literals and -1].  The
+// block's space is reserved only if all of it fits, so the dump stays parseable.
+__device__ void dump_learned(Ctx& J, const SolveArgs& a, int32_t inst)
+{
+    const int lane = lane_id();
+    int32_t s = 0;
+    for (int32_t i = lane; i < J.nlearnt; i += kWave) s += J.lmeta[4 * i + 1] + 1;
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+    const int32_t n0 = J.dl > 0 ? uni(J.tlim[1]) : J.tsize;   // level-0 part of the trail
+    const int64_t slots = (int64_t)uni(s) + 2 * (int64_t)n0;
+    const unsigned long long total = (unsigned long long)(slots + 2);
+    long long base = -1;
+    if (lane == 0) {
+        atomicAdd(a.learned_need, total);
+        unsigned long long old = __hip_atomic_load(a.learned_pos, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        while (old + total <= (unsigned long long)a.learned_cap) {
+            const unsigned long long prev = atomicCAS(a.learned_pos, old, old + total);
+            if (prev == old) {
+                base = (long long)old;
+                break;
+            }
+            old = prev;
+        }
+    }
+    base = __shfl(base, 0);
+    if (base < 0) return;
+    int32_t* out = a.learned_out + base;
+    if (lane == 0) {
+        out[0] = inst;
+        out[1] = (int32_t)slots;
+    }
+    int64_t p = 2;
+    for (int32_t i = 0; i < J.nlearnt; i++) {
+        const int32_t st = uni(J.lmeta[4 * i]), len = uni(J.lmeta[4 * i + 1]);
+        for (int32_t k = lane; k < len; k += kWave) out[p + k] = J.larena[st + k];
+        if (lane == 0) out[p + len] = -1;
+        p += len + 1;
+    }
+    for (int32_t t = lane; t < n0; t += kWave) {
+        out[p + 2 * t] = J.trail[t];
+        out[p + 2 * t + 1] = -1;
+    }
+}
+
 // Returns the job verdict: 0 SAT, 1 UNSAT, 2 UNDEF (budget), 3 skipped (instance decided).
 // deadline: wall-clock tick after which the job stops as UNDEF (0: none).
 __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_t max_conflicts, uint64_t deadline)
@@ -917,6 +962,7 @@ void k_solve(SolveArgs a)
         wave_mem_sync();
         const int32_t inst = jd.inst;
         if (a.o.inst_conflicts > 0 && J.ran) (void)report_conflicts(J, a, inst);
+        if (a.learned_out && J.ran && J.setup_ok) dump_learned(J, a, inst);
         if (st == 0) {
             int32_t won = 0;
             if (lane == 0) won = atomicCAS(&a.inst_status[inst], 2, 0) == 2;

@@ -77,6 +77,11 @@ struct SolveArgs {
     int32_t* ws;                    // global workspace
     int64_t ws_ints_per_wave;
     int64_t lds_per_wave;           // bytes of each wavefront's LDS region
+    // learned-clause dump (testing aid; all null otherwise)
+    int32_t* learned_out;
+    int64_t learned_cap;
+    unsigned long long* learned_pos;    // slots reserved by blocks that fit
+    unsigned long long* learned_need;   // slots all blocks asked for
     // per-wave clause-part layout (ints), identical in LDS and global mode
     int32_t wo_cap, wl_cap, wpair_cap, lmeta_cap;
     int32_t max_n;

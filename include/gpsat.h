@@ -198,6 +198,12 @@ typedef struct gpsat_solve_opts {
     int32_t telemetry;       /* 1: per-phase times in job_stats (set-up, BCP, analysis, backtrack,
                                 decide), the phases of RuntimeStatistics (src/Statistics/
                                 RuntimeStatistics.cu:246-360); 0: those fields stay 0 */
+    int32_t *learned_out;    /* optional host buffer (testing aid; no reference counterpart): at
+                                the end of each job that ran, its held learned clauses and its
+                                level-0 literals (as unit clauses), as blocks [instance, slots,
+                                then per clause its literal codes and -1]; each is implied by the
+                                formula and the job's cube.  A block that does not fit is left out */
+    int64_t learned_cap;     /* int32 slots of learned_out */
 } gpsat_solve_opts;
 
 typedef struct gpsat_solve_info {
@@ -210,6 +216,8 @@ typedef struct gpsat_solve_info {
     int32_t n_sat, n_unsat, n_undef, n_verified, n_errors;
     int32_t epochs;          /* graph replays in epoch mode (1 otherwise) */
     int32_t chunk;           /* jobs per queue grab that was used */
+    int64_t learned_len;     /* slots written to opts.learned_out */
+    int64_t learned_need;    /* slots all blocks needed (> learned_len: some were left out) */
 } gpsat_solve_info;
 
 int gpsat_solve_opts_default(gpsat_solve_opts *o);

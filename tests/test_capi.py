@@ -41,3 +41,31 @@ def test_no_device_is_an_error_not_a_fallback():
         pytest.skip("a GPU is visible")
     with pytest.raises(G.GpsatError):
         G.GpuBatch(G.Batch.from_formulas([G.Formula.from_clauses(2, [[1, 3]])]))
+
+
+def test_struct_mirrors_match_the_header(tmp_path):
+    """The ctypes mirrors of gpsat_solve_opts / gpsat_solve_info have the C layout (size and
+    every field offset), checked against the header compiled with the host compiler."""
+    import shutil
+    import subprocess
+    cc = shutil.which("gcc") or shutil.which("cc")
+    if cc is None:
+        pytest.skip("no host C compiler")
+    lines = []
+    for cname, py in (("gpsat_solve_opts", G.SolveOpts), ("gpsat_solve_info", G.SolveInfo)):
+        lines.append(f'printf("{cname} size %zu\\n", sizeof({cname}));')
+        for f, _ in py._fields_:
+            lines.append(f'printf("{cname} {f} %zu\\n", offsetof({cname}, {f}));')
+    src = tmp_path / "layout.c"
+    src.write_text("#include <stdio.h>\n#include <stddef.h>\n#include <stdint.h>\n"
+                   f'#include "{G.HEADER_PATH}"\nint main(void) {{\n' + "\n".join(lines) + "\nreturn 0;\n}\n")
+    exe = tmp_path / "layout"
+    subprocess.run([cc, "-o", str(exe), str(src)], check=True)
+    got = {}
+    for ln in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.splitlines():
+        cname, field, val = ln.split()
+        got[(cname, field)] = int(val)
+    for cname, py in (("gpsat_solve_opts", G.SolveOpts), ("gpsat_solve_info", G.SolveInfo)):
+        assert got[(cname, "size")] == ctypes.sizeof(py), cname
+        for f, _ in py._fields_:
+            assert got[(cname, f)] == getattr(py, f).offset, (cname, f)

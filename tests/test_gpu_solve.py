@@ -8,7 +8,7 @@ import pytest
 
 import gpupsat_amd as G
 from gpupsat_amd import workloads as W
-from gpupsat_amd.formula import Batch, read_dimacs
+from gpupsat_amd.formula import Batch, Formula, read_dimacs
 
 from conftest import GOLDEN
 
@@ -254,3 +254,41 @@ def test_telemetry_phases(orc):
     busy = js1[:, 3] > 0
     assert (ph[busy, 1] > 0).all()                       # BCP time where there were propagations
     assert (ph[js1[:, 1] > 0, 2] > 0).all()              # analysis time where there were conflicts
+
+
+def test_learned_clauses_are_implied(orc):
+    """Soundness of learning: every learned clause a job still holds at its end, and every
+    level-0 literal, is implied by the formula -- the formula plus the clause's negation as
+    unit clauses is UNSAT for the oracle's CDCL.  A small learned DB (16 clauses) forces
+    reductions; random two-literal clauses over the SAT instances show the check can fail."""
+    print("[solve] learned clauses implied: 40 x random 3-SAT n=50, learnt_cap 16", flush=True)
+    b = W.random_3sat_batch(40, 50, 213, seed=77)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(learnt_cap=16), learned_cap=1 << 20)
+    info = res["info"]
+    assert info["learned_len"] == info["learned_need"] > 0
+    blocks = res["learned"]
+    assert sorted(i for i, _ in blocks) == list(range(b.n_inst))
+    checked = units = not_implied = 0
+    rng = np.random.default_rng(3)
+    for inst, clauses in blocks:
+        f = b.instance(inst)
+        base = f.clauses()
+        assert all(len(c) >= 1 for c in clauses)
+        # up to 12 learned clauses per instance, and every level-0 unit
+        long_ = [c for c in clauses if len(c) > 1]
+        picks = [long_[k] for k in rng.permutation(len(long_))[:12]] + [c for c in clauses if len(c) == 1]
+        for c in picks:
+            g = Formula.from_clauses(f.n_vars, base + [[l ^ 1] for l in c])
+            assert truth(orc, g) == G.UNSAT, (inst, c)
+            checked += 1
+            units += len(c) == 1
+        if res["status"][inst] == G.SAT:
+            for _ in range(4):
+                v = rng.choice(f.n_vars, size=2, replace=False)
+                c = [2 * int(x) + int(rng.integers(2)) for x in v]
+                g = Formula.from_clauses(f.n_vars, base + [[l ^ 1] for l in c])
+                not_implied += truth(orc, g) == G.SAT
+    assert checked > 100 and not_implied > 0
+    print(f"[solve] {checked} learned clauses checked ({units} level-0 units); "
+          f"{not_implied} random clauses correctly found not implied", flush=True)
