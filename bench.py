@@ -277,7 +277,7 @@ def main():
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the instance batch would shard across GPUs with no "
-              "exchange (see DESIGN.md); multi-GPU runs are not part of this build.", file=sys.stderr)
+              "exchange between them, and multi-GPU runs are not measured here.", file=sys.stderr)
         sys.exit(2)
 
     import gpupsat_amd as G

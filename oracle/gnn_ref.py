@@ -1,6 +1,6 @@
 """CPU oracle of the learned branching network (TEST INFRASTRUCTURE ONLY).
 
-No reference counterpart: the network is this build's addition.  This module restates
+No reference counterpart: the network has no equivalent in the reference.  This module restates
 the network's mathematics in numpy FP64 so the GPU kernels (gpupsat_amd/csrc/gnn.hip)
 can be checked: forward values and analytic gradients, with the GPU's BF16 rounding
 points reproduced when quant=True.  Its own backward is pinned by central finite
