@@ -194,7 +194,7 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
     d.h_coff.resize(M + n_inst);
     d.h_lits.assign(lits ? lits + l0 : nullptr, lits ? lits + l0 + L : nullptr);
     int64_t wbase = 0;
-    bool fast3 = true;
+    bool fast3 = true, every3 = true;
     for (int32_t i = 0; i < n_inst; i++) {
         const int64_t cb = inst_clause_off[i], ce = inst_clause_off[i + 1];
         if (ce < cb || n_vars[i] < 0) {
@@ -228,12 +228,14 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
         bool all3 = true;
         for (int64_t c = cb; c < ce && all3; c++) all3 = clause_off[c + 1] - clause_off[c] == 3;
         fast3 = fast3 && all3 && fast3_fits(mi.n, mi.m);
+        every3 = every3 && all3;
         wbase += words_per_row(mi.n);
         d.max_n = std::max(d.max_n, mi.n);
         d.max_m = std::max(d.max_m, mi.m);
     }
     d.total_words = wbase;
     d.fast3 = fast3 && n_inst > 0;
+    d.all3 = every3 && n_inst > 0;
     hipError_t e = hipGetDevice(&d.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
     if (e == hipSuccess && n_inst) e = hipMalloc(&d.meta, sizeof(InstMeta) * n_inst);

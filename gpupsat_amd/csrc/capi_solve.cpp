@@ -180,6 +180,7 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     a.o.inst_conflicts = o.inst_conflicts;
     a.o.telemetry = o.telemetry ? 1 : 0;
     a.max_n = d.max_n;
+    a.all3 = d.all3 ? 1 : 0;
     a.wo_cap = max_L;
     a.wl_cap = 2 * d.max_n * a.o.watch_slack;
     a.wpair_cap = 2 * (d.max_m + o.learnt_cap);

@@ -19,6 +19,7 @@ struct DeviceBatch {
     int64_t n_lits = 0;
     int64_t total_words = 0;      // sum of W_i (rows buffer holds A * total_words words)
     bool fast3 = false;           // every instance fits the register-resident 3-literal kernel
+    bool all3 = false;            // every clause of every instance has exactly 3 literals
     InstMeta* meta = nullptr;     // device [n_inst]
     int32_t* lits = nullptr;      // device [n_lits]
     int32_t* coff = nullptr;      // device [n_clauses + n_inst], per-instance relative offsets

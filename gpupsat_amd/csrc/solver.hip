@@ -75,6 +75,7 @@ __device__ __forceinline__ void wave_mem_sync()
 struct Ctx {
     // instance (uniform)
     int32_t n, m, inst;
+    bool all3;           // original clause c is lits[3c .. 3c+3)
     const int32_t* lits;
     const int32_t* coff;
     // per-var / per-literal state (LDS)
@@ -118,7 +119,10 @@ struct Ctx {
 
 __device__ __forceinline__ void clause_span(const Ctx& J, int32_t c, const int32_t*& cl, int32_t& len)
 {
-    if (c < J.m) {
+    if (c < J.m && J.all3) {
+        cl = J.lits + 3 * c;
+        len = 3;
+    } else if (c < J.m) {
         const int32_t b = J.coff[c];
         len = J.coff[c + 1] - b;
         cl = J.lits + b;
@@ -167,15 +171,24 @@ __device__ int32_t walk_list(Ctx& J, int32_t fl, int32_t* W, int32_t base, int32
                 if (w0 != fl && w1 != fl) {
                     kept = false;   // stale entry: the clause moved its watch away from fl
                 } else if (J.lv[other] != 1) {
-                    const int32_t* cl;
-                    int32_t len;
-                    clause_span(J, c, cl, len);
                     int32_t rep = -1;
-                    for (int32_t k = 0; k < len; k++) {
-                        const int32_t q = cl[k];
-                        if (q != w0 && q != w1 && J.lv[q] != -1) {
-                            rep = q;
-                            break;
+                    if (!LEARNT && J.all3) {
+                        // three literals, two of them the watches: the candidate is the third
+                        // (no offset reads, no search loop)
+                        const int32_t* cl = J.lits + 3 * c;
+                        const int32_t q = (int32_t)((uint32_t)cl[0] + (uint32_t)cl[1] + (uint32_t)cl[2] -
+                                                    (uint32_t)w0 - (uint32_t)w1);
+                        if (q != w0 && q != w1 && J.lv[q] != -1) rep = q;
+                    } else {
+                        const int32_t* cl;
+                        int32_t len;
+                        clause_span(J, c, cl, len);
+                        for (int32_t k = 0; k < len; k++) {
+                            const int32_t q = cl[k];
+                            if (q != w0 && q != w1 && J.lv[q] != -1) {
+                                rep = q;
+                                break;
+                            }
                         }
                     }
                     if (rep >= 0) {
@@ -594,6 +607,7 @@ __device__ bool setup_instance(Ctx& J, const SolveArgs& a, int32_t inst)
     J.m = uni(mi.m);
     J.lits = a.lits + mi.lit_base;
     J.coff = a.coff + mi.coff_base;
+    J.all3 = a.all3 != 0;
     const int32_t n = J.n, m = J.m, n2 = 2 * n;
     const int64_t vb = a.var_base[inst];
     J.slack = o.watch_slack;

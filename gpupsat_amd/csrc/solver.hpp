@@ -85,6 +85,7 @@ struct SolveArgs {
     // per-wave clause-part layout (ints), identical in LDS and global mode
     int32_t wo_cap, wl_cap, wpair_cap, lmeta_cap;
     int32_t max_n;
+    int32_t all3;                   // every original clause has 3 literals (clause c at lits + 3c)
     SolveOptsDev o;
 };
 
