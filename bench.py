@@ -346,6 +346,21 @@ def main():
     props_lo = d_props.to_host(np.int32, N * A)
     conf_lo = d_conf.to_host(np.int32, N * A)
 
+    # the fused launch over the trail densities the low-density sweep asks for
+    sweep = {}
+    for pd in (0.02, 0.05, 0.1, 0.3):
+        r_p = W.random_assignments(b, A, pd, seed=2)
+        d_p = G.DevBuf.from_host(r_p)
+        gb.eval_bcp_dev(d_p, A, d_counts, d_out, d_conf, d_props)
+        ms_p = timed(lambda: gb.eval_bcp_dev(d_p, A, d_counts, d_out, d_conf, d_props), args.steps)
+        cf = d_conf.to_host(np.int32, N * A)
+        pr = d_props.to_host(np.int32, N * A)
+        gbs = N * algorithmic_bytes_per_instance(n, m, L, A, True, True) / (ms_p * 1e-3) / 1e9
+        sweep[f"p{pd}"] = {"fused_ms": ms_p, "instances_per_s": N / (ms_p * 1e-3), "algorithmic_gbs": gbs,
+                           "frac_of_8tbs": gbs / 8000.0, "conflict_frac": float(cf.mean()),
+                           "implied_literals_per_s": float(pr.sum()) / (ms_p * 1e-3)}
+        d_p.close()
+
     # practical HBM peak: streaming copy of 2 GiB
     cb = 1 << 31
     src, dst = G.DevBuf(cb), G.DevBuf(cb)
@@ -434,6 +449,7 @@ def main():
             "p0.05_ms": ms_bcp_lo,
             "p0.05_conflict_frac": float(conf_lo.mean()),
             "p0.05_props_per_s": float(props_lo.sum()) / (ms_bcp_lo * 1e-3),
+            "fused_density_sweep": sweep,
         },
         "parity_sample_ok": parity,
         "legs": legs,
