@@ -69,3 +69,22 @@ def test_struct_mirrors_match_the_header(tmp_path):
         assert got[(cname, "size")] == ctypes.sizeof(py), cname
         for f, _ in py._fields_:
             assert got[(cname, f)] == getattr(py, f).offset, (cname, f)
+
+
+def test_integration_example_compiles(tmp_path):
+    """The C++ caller shown in INTEGRATION.md is type-correct against include/gpsat.h (compiled
+    with the host compiler inside a function that declares its inputs; not run)."""
+    import shutil
+    import subprocess
+    cxx = shutil.which("g++") or shutil.which("c++")
+    if cxx is None:
+        pytest.skip("no host C++ compiler")
+    text = open(os.path.join(os.path.dirname(G.HEADER_PATH), "..", "INTEGRATION.md")).read()
+    block = re.search(r"```cpp\n(.*?)```", text, flags=re.S).group(1)
+    body = "\n".join(l for l in block.splitlines() if not l.startswith("#include"))
+    src = tmp_path / "example.cpp"
+    src.write_text('#include <cstdio>\n#include <cstdint>\n#include <vector>\n#include "gpsat.h"\n'
+                   "int example(const int32_t* lits, const int64_t* clause_off, int64_t n_clauses, int32_t n_vars)\n"
+                   "{\n" + body + "\nreturn status;\n}\n")
+    subprocess.run([cxx, "-std=c++17", "-fsyntax-only", "-Wall", "-I", os.path.dirname(G.HEADER_PATH), str(src)],
+                   check=True)
