@@ -915,7 +915,7 @@ void k_solve(SolveArgs a)
     extern __shared__ __align__(16) char smem_all[];
     const int lane = lane_id();
     // the wavefronts of a workgroup are independent (separate jobs, private LDS regions,
-    // wave-level ordering only): several per workgroup let a CU hold more than 16
+    // wave-level ordering only)
     const int wslot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x / kWave) + wslot;
     char* smem = smem_all + (size_t)wslot * a.lds_per_wave;
