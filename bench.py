@@ -117,7 +117,7 @@ def solve_leg(G, W, b, cpu):
         "verdict_agreement_sample": f"{agree}/{len(idx)}",
         "mean_conflicts": float(js[:, 1].mean()), "mean_props": float(js[:, 3].mean()),
         "props_per_s": float(js[:, 3].sum()) / (inf["kernel_ms"] * 1e-3),
-        "waves": inf["waves"], "clause_mem": "LDS" if inf["clause_mem"] == 1 else "HBM",
+        "waves": inf["waves"], "clause_mem": {1: "LDS", 2: "HBM", 3: "split"}[inf["clause_mem"]],
     }
     # the reference's own sequential search on the GPU (trajectory mode), same instances
     gb.solve_ref(max_iterations=0, learned_cap=2048, learned_lits=16384)
@@ -202,7 +202,7 @@ def mixed_leg(G, W, N: int, deadline_ms: float, pool: int):
         "kernel_ms": ms, "epochs": inf["epochs"], "jobs": int(len(jobs[0])),
         "jobs_finished": int((js[:, 0] != 2).sum()),
         "props_per_s": float(js[:, 3].sum()) / (ms * 1e-3), "conflicts_per_s": float(js[:, 1].sum()) / (ms * 1e-3),
-        "launch_shape": {"waves": inf["waves"], "clause_mem": "LDS" if inf["clause_mem"] == 1 else "HBM",
+        "launch_shape": {"waves": inf["waves"], "clause_mem": {1: "LDS", 2: "HBM", 3: "split"}[inf["clause_mem"]],
                          "chunk": inf["chunk"], "autotune_s": tune_s},
         "autotune_report": rep.strip().splitlines(),
     }

@@ -96,6 +96,8 @@ Sample sample_jobs(const int32_t* job_inst, const int32_t* cube_off, const int32
 
 extern "C" {
 
+static const char* mem_name(int mem) { return mem == 1 ? "LDS" : mem == 3 ? "split" : "HBM"; }
+
 int gpsat_autotune_solve(gpsat_batch* b, const gpsat_solve_opts* base, const int32_t* job_inst, const int32_t* cube_off,
                          const int32_t* cube_lits, int32_t n_jobs, int32_t sample_jobs_max, double trial_ms,
                          gpsat_solve_opts* best, char* report, int32_t report_len)
@@ -151,7 +153,7 @@ int gpsat_autotune_solve(gpsat_batch* b, const gpsat_solve_opts* base, const int
             done += stats[(size_t)j * GPSAT_JOB_STATS] != 2;
         }
         const double rate = inf.kernel_ms > 0 ? (double)props / inf.kernel_ms : 0.0;
-        rep.line("  mem=%s waves=%d chunk=%d: %.0f props/ms, %lld/%d jobs finished, %.1f ms", mem == 1 ? "LDS" : "HBM",
+        rep.line("  mem=%s waves=%d chunk=%d: %.0f props/ms, %lld/%d jobs finished, %.1f ms", mem_name(mem),
                  inf.waves, inf.chunk, rate, (long long)done, J, inf.kernel_ms);
         return rate;
     };
@@ -159,7 +161,7 @@ int gpsat_autotune_solve(gpsat_batch* b, const gpsat_solve_opts* base, const int
     // clause memory x wavefronts per CU (chunk automatic), then the chunk
     double best_rate = -1.0;
     int32_t bm = 2, bw = 0, bc = 0;
-    for (int32_t mem : {1, 2}) {
+    for (int32_t mem : {1, 2, 3}) {
         for (int32_t per_cu : {8, 16, 24, 32}) {
             const double r = trial(mem, per_cu * cus, 0);
             if (r > best_rate) {
@@ -187,7 +189,7 @@ int gpsat_autotune_solve(gpsat_batch* b, const gpsat_solve_opts* base, const int
     best->clause_mem = bm;
     best->waves = bw;
     best->chunk = bc;
-    rep.line("chosen: mem=%s waves=%d chunk=%s (%.0f props/ms)", bm == 1 ? "LDS" : "HBM", bw,
+    rep.line("chosen: mem=%s waves=%d chunk=%s (%.0f props/ms)", mem_name(bm), bw,
              bc ? std::to_string(bc).c_str() : "auto", best_rate);
     rep.copy_to(report, report_len);
     return GPSAT_OK;

@@ -194,11 +194,16 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     HIP_OR(hipGetDeviceProperties(&prop, d.device));
     const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
     const size_t lds_block_max = prop.sharedMemPerBlock ? prop.sharedMemPerBlock : 65536;
-    bool use_lds;
-    if (o.clause_mem == 1) use_lds = lds_all <= lds_block_max;
-    else if (o.clause_mem == 2) use_lds = false;
-    else use_lds = lds_all <= std::min<size_t>(48 * 1024, lds_block_max);
-    const size_t lds_bytes = use_lds ? lds_all : state_b;
+    // clause part: 1 all in LDS, 2 all in HBM, 3 the original clauses' watch lists and
+    // watch pairs in LDS, the learned clauses' storage in HBM
+    const size_t lds_hyb = state_b + 4 * (size_t)(a.wo_cap + a.wpair_cap - 2 * o.learnt_cap);
+    int mode;
+    if (o.clause_mem == 1) mode = lds_all <= lds_block_max ? 1 : 2;
+    else if (o.clause_mem == 2) mode = 2;
+    else if (o.clause_mem == 3) mode = lds_hyb <= lds_block_max ? 3 : 2;
+    else mode = lds_all <= std::min<size_t>(48 * 1024, lds_block_max) ? 1 : 2;
+    const bool use_lds = mode == 1;
+    const size_t lds_bytes = mode == 1 ? lds_all : mode == 3 ? lds_hyb : state_b;
     if (lds_bytes > lds_block_max) {
         set_error("instance too large for the per-wavefront solver state in LDS");
         return GPSAT_ERR_CAPACITY;
@@ -352,7 +357,7 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     HIP_OR(hipEventCreate(&e2));
     HIP_OR(hipEventRecord(e0, st));
     if (!epochs) {
-        HIP_OR(launch_solve(a, use_lds, waves, lds_bytes, st));
+        HIP_OR(launch_solve(a, mode, waves, lds_bytes, st));
     } else {
         // one restart epoch = one replay of this graph: empty the queue, run the persistent
         // solve over the open jobs, re-queue the stopped ones, publish how many remain
@@ -363,7 +368,7 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
         mem.ptrs.push_back(d_remaining_ptr);
         HIP_OR(hipStreamBeginCapture(st, hipStreamCaptureModeThreadLocal));
         hipError_t ce = launch_epoch_begin(a, st);
-        if (ce == hipSuccess) ce = launch_solve(a, use_lds, waves, lds_bytes, st);
+        if (ce == hipSuccess) ce = launch_solve(a, mode, waves, lds_bytes, st);
         if (ce == hipSuccess) ce = launch_epoch_end(a, d_remaining_ptr, st);
         if (ce == hipSuccess) ce = hipMemcpyAsync(h_remaining, d_remaining_ptr, sizeof(int32_t), hipMemcpyDeviceToHost, st);
         hipError_t ee = hipStreamEndCapture(st, &graph);
@@ -435,7 +440,7 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     inf.kernel_ms = ms_solve;
     inf.verify_ms = ms_verify;
     inf.waves = waves;
-    inf.clause_mem = use_lds ? 1 : 2;
+    inf.clause_mem = mode;
     inf.lds_bytes = (int64_t)lds_bytes;
     inf.workspace_bytes = use_lds ? 0 : (int64_t)waves * a.ws_ints_per_wave * 4;
     inf.epochs = epochs_run;

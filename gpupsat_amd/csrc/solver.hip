@@ -96,7 +96,8 @@ struct Ctx {
     // clause part (LDS or HBM)
     int32_t* Wo;         // original-clause watch lists (capacity = occurrences)
     int32_t* Wl;         // learned-clause watch lists, slack slots per literal
-    int32_t* wpair;      // [2*(m + learnt_cap)] the two watched literals of every clause
+    int32_t* wpair;      // [2*m] the two watched literals of every original clause
+    int32_t* wpl;        // [2*learnt_cap] the same for the learned clauses (clause m + i at 2i)
     int32_t* lmeta;      // [4*learnt_cap] start, len, flags, lbd; then [learnt_cap] remap slots
     int32_t* larena;     // learned literals
     int32_t slack, learnt_cap, larena_cap;
@@ -166,7 +167,8 @@ __device__ int32_t walk_list(Ctx& J, int32_t fl, int32_t* W, int32_t base, int32
             if (dead) {
                 kept = false;
             } else {
-                const int32_t w0 = J.wpair[2 * c], w1 = J.wpair[2 * c + 1];
+                int32_t* wp = LEARNT ? J.wpl + 2 * (c - J.m) : J.wpair + 2 * c;
+                const int32_t w0 = wp[0], w1 = wp[1];
                 const int32_t other = (w0 == fl) ? w1 : w0;
                 if (w0 != fl && w1 != fl) {
                     kept = false;   // stale entry: the clause moved its watch away from fl
@@ -196,14 +198,14 @@ __device__ int32_t walk_list(Ctx& J, int32_t fl, int32_t* W, int32_t base, int32
                         if (!LEARNT) {
                             const int32_t pos = atomicAdd(&J.wcnt_o[rep], 1);
                             J.Wo[J.wstart[rep] + pos] = c;
-                            J.wpair[2 * c] = other;
-                            J.wpair[2 * c + 1] = rep;
+                            wp[0] = other;
+                            wp[1] = rep;
                         } else {
                             const int32_t pos = atomicAdd(&J.wcnt_l[rep], 1);
                             if (pos < J.slack) {
                                 J.Wl[rep * J.slack + pos] = c;
-                                J.wpair[2 * c] = other;
-                                J.wpair[2 * c + 1] = rep;
+                                wp[0] = other;
+                                wp[1] = rep;
                             } else {
                                 // no room: the learned clause stops propagating (it may still
                                 // be a reason; its literals stay in the arena)
@@ -377,8 +379,8 @@ __device__ int32_t store_learned(Ctx& J, int32_t len, int32_t lbd)
 // Watch a learned clause on literals a and b (uniform).
 __device__ void watch_learned(Ctx& J, int32_t c, int32_t a, int32_t b)
 {
-    J.wpair[2 * c] = a;
-    J.wpair[2 * c + 1] = b;
+    J.wpl[2 * (c - J.m)] = a;
+    J.wpl[2 * (c - J.m) + 1] = b;
     const int32_t pa = uni(J.wcnt_l[a]);
     const int32_t pb = uni(J.wcnt_l[b]) + (a == b ? 1 : 0);
     if (pa < J.slack && pb < J.slack) {
@@ -456,8 +458,8 @@ __device__ void reduce_db(Ctx& J, int mode)
             len = J.lmeta[4 * i + 1];
             const int32_t fl = J.lmeta[4 * i + 2];
             lbd = J.lmeta[4 * i + 3];
-            w0 = J.wpair[2 * (m + i)];
-            w1 = J.wpair[2 * (m + i) + 1];
+            w0 = J.wpl[2 * i];
+            w1 = J.wpl[2 * i + 1];
             del = (fl & F_DELETED) != 0;
             const bool wanted = mode == 0 ? (lbd <= 3 || (fl & F_USED)) : mode == 1 ? lbd <= 3 : false;
             keep = (fl & F_LOCKED) || (!del && wanted);
@@ -471,8 +473,8 @@ __device__ void reduce_db(Ctx& J, int mode)
             J.lmeta[4 * ni + 1] = len;
             J.lmeta[4 * ni + 2] = del ? F_DELETED : 0;
             J.lmeta[4 * ni + 3] = lbd;
-            J.wpair[2 * (m + ni)] = w0;
-            J.wpair[2 * (m + ni) + 1] = w1;
+            J.wpl[2 * ni] = w0;
+            J.wpl[2 * ni + 1] = w1;
         }
         ns += popc64(km);
         wave_mem_sync();
@@ -502,7 +504,7 @@ __device__ void reduce_db(Ctx& J, int mode)
     for (int32_t i = lane; i < ns; i += kWave) {
         if (J.lmeta[4 * i + 2] & F_DELETED) continue;
         const int32_t c = m + i;
-        const int32_t wl[2] = {J.wpair[2 * c], J.wpair[2 * c + 1]};
+        const int32_t wl[2] = {J.wpl[2 * i], J.wpl[2 * i + 1]};
         for (int h = 0; h < 2; h++) {
             const int32_t p = atomicAdd(&J.wcnt_l[wl[h]], 1);
             if (p < J.slack) {
@@ -590,7 +592,9 @@ __device__ void carve_clauses(Ctx& J, int32_t* c, const SolveArgs& a)
 {
     J.Wo = c;      c += a.wo_cap;
     J.Wl = c;      c += a.wl_cap;
-    J.wpair = c;   c += a.wpair_cap;
+    J.wpair = c;
+    J.wpl = c + (a.wpair_cap - 2 * a.o.learnt_cap);
+    c += a.wpair_cap;
     J.lmeta = c;   c += a.lmeta_cap;
     J.larena = c;
 }
@@ -908,7 +912,11 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
 
 }  // namespace
 
-template <bool kLds>
+// kMode: where the clause part of a job's state sits -- 1 all in LDS, 2 all in this
+// wavefront's slice of the HBM workspace, 3 the watch lists and watch pairs of the original
+// clauses in LDS (read on every propagation step), the learned clauses' lists, pairs,
+// metadata and literals in HBM.
+template <int kMode>
 __global__ __launch_bounds__(kSolveWavesPerBlock * kWave) __attribute__((amdgpu_waves_per_eu(kSolveWavesPerEu, kSolveWavesPerEu)))
 void k_solve(SolveArgs a)
 {
@@ -921,9 +929,14 @@ void k_solve(SolveArgs a)
     char* smem = smem_all + (size_t)wslot * a.lds_per_wave;
     Ctx J;
     carve_state(J, smem, a.max_n);
-    int32_t* cpart = kLds ? (int32_t*)(smem + Layout::state_bytes(a.max_n))
-                          : a.ws + gwave * a.ws_ints_per_wave;
+    int32_t* cpart = kMode == 1 ? (int32_t*)(smem + Layout::state_bytes(a.max_n))
+                                : a.ws + gwave * a.ws_ints_per_wave;
     carve_clauses(J, cpart, a);
+    if (kMode == 3) {
+        int32_t* l = (int32_t*)(smem + Layout::state_bytes(a.max_n));
+        J.Wo = l;
+        J.wpair = l + a.wo_cap;
+    }
     J.setup_inst = -1;
     J.setup_ok = false;
     J.n = 0;
@@ -1082,16 +1095,18 @@ hipError_t launch_epoch_end(const SolveArgs& a, int32_t* remaining, hipStream_t 
 
 size_t solve_state_lds_bytes(int32_t max_n) { return Layout::state_bytes(max_n); }
 
-hipError_t launch_solve(const SolveArgs& a, bool lds_clauses, int waves, size_t lds_bytes, hipStream_t st)
+hipError_t launch_solve(const SolveArgs& a, int clause_mem, int waves, size_t lds_bytes, hipStream_t st)
 {
     // waves is a multiple of the wavefronts per workgroup (solve_waves_per_block)
     const int wpb = solve_waves_per_block(lds_bytes);
     if (waves % wpb != 0 || (size_t)a.lds_per_wave != lds_bytes) return hipErrorInvalidValue;
     const dim3 grid(waves / wpb), block(kWave * wpb);
-    if (lds_clauses)
-        hipLaunchKernelGGL(k_solve<true>, grid, block, lds_bytes * wpb, st, a);
+    if (clause_mem == 1)
+        hipLaunchKernelGGL(k_solve<1>, grid, block, lds_bytes * wpb, st, a);
+    else if (clause_mem == 3)
+        hipLaunchKernelGGL(k_solve<3>, grid, block, lds_bytes * wpb, st, a);
     else
-        hipLaunchKernelGGL(k_solve<false>, grid, block, lds_bytes * wpb, st, a);
+        hipLaunchKernelGGL(k_solve<2>, grid, block, lds_bytes * wpb, st, a);
     return hipGetLastError();
 }
 

@@ -120,7 +120,7 @@ hipError_t launch_solve_ref(const RefArgs& a, hipStream_t st);
 size_t solve_state_lds_bytes(int32_t max_n);
 hipError_t launch_epoch_begin(const SolveArgs& a, hipStream_t st);
 hipError_t launch_epoch_end(const SolveArgs& a, int32_t* remaining, hipStream_t st);
-hipError_t launch_solve(const SolveArgs& a, bool lds_clauses, int waves, size_t lds_bytes, hipStream_t st);
+hipError_t launch_solve(const SolveArgs& a, int clause_mem, int waves, size_t lds_bytes, hipStream_t st);
 int solve_waves_per_block(size_t lds_bytes);
 
 }  // namespace gpsat

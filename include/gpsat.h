@@ -177,7 +177,9 @@ typedef struct gpsat_solve_opts {
     double max_ms;           /* per-job wall-clock budget in ms (<= 0: none), exceeded -> UNDEF */
     int32_t waves;           /* persistent wavefronts, rounded up to whole workgroups of 4
                                 (0: automatic, up to 32 per CU as LDS allows) */
-    int32_t clause_mem;      /* clause part of the job state: 0 automatic, 1 LDS, 2 HBM */
+    int32_t clause_mem;      /* clause part of the job state: 0 automatic, 1 LDS, 2 HBM, 3 the
+                                original clauses' watch lists and pairs in LDS, the learned
+                                clauses' storage in HBM */
     int32_t verify;          /* 1: re-check every SAT model with the clause-evaluation kernel */
     int32_t chunk;           /* jobs per queue grab (0: automatic); cubes of one instance in one
                                 chunk reuse its watch lists on the wavefront */
@@ -210,7 +212,7 @@ typedef struct gpsat_solve_info {
     double kernel_ms;        /* solve kernel (HIP events on the batch stream) */
     double verify_ms;        /* model re-verification kernel */
     int32_t waves;
-    int32_t clause_mem;      /* 1 LDS, 2 HBM */
+    int32_t clause_mem;      /* 1 LDS, 2 HBM, 3 split (see gpsat_solve_opts) */
     int64_t lds_bytes;       /* per wavefront */
     int64_t workspace_bytes;
     int32_t n_sat, n_unsat, n_undef, n_verified, n_errors;
@@ -253,7 +255,7 @@ int gpsat_solve_ref(gpsat_batch *b, const uint8_t *dead, int32_t max_iterations,
 /* ---- launch-shape autotuner (no reference counterpart: the reference takes -b/-t and
  * fixes its strategies at compile time, src/SATSolver/Configs.cuh).
  * gpsat_autotune_solve: short trial solves of a sample of the jobs (whole instances, at
- * most sample_jobs jobs; 0: 4096) under each launch shape -- clause part in LDS or HBM,
+ * most sample_jobs jobs; 0: 4096) under each launch shape -- clause part in LDS, HBM or split,
  * 8/16/24/32 persistent wavefronts per CU, then 1/8/32/64 jobs per queue grab when there
  * are cubes -- each with deadline trial_ms (0: 300).  *best = *base with waves,
  * clause_mem and chunk of the shape with the most propagations per ms of kernel time.

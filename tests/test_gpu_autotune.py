@@ -42,7 +42,7 @@ def test_autotune_solve_same_verdicts(orc):
     with G.GpuBatch(b) as gb:
         best, rep = gb.autotune_solve(base, jobs=jobs, trial_ms=150.0)
         assert "chosen:" in rep
-        assert best.waves > 0 and best.clause_mem in (1, 2)
+        assert best.waves > 0 and best.clause_mem in (1, 2, 3)
         assert best.max_conflicts == 3000 and best.job_order == 1
         r1 = gb.solve(base, jobs=jobs)
         r2 = gb.solve(best, jobs=jobs)

@@ -48,7 +48,7 @@ def test_reference_fixtures(orc):
             check_models(orc, gb, res, b)
 
 
-@pytest.mark.parametrize("mem", [1, 2])
+@pytest.mark.parametrize("mem", [1, 2, 3])
 def test_3sat_n50_verdicts(orc, mem):
     print(f"[solve] 100 x n=50 m=213 clause_mem={mem}", flush=True)
     b = W.random_3sat_batch(100, 50, 213, seed=0)
@@ -69,13 +69,14 @@ def test_reference_decision_rule(orc):
     assert res["status"].tolist() == [truth(orc, b.instance(i)) for i in range(b.n_inst)]
 
 
-def test_tiny_learned_db_and_watch_slack(orc):
+@pytest.mark.parametrize("mem", [0, 3])
+def test_tiny_learned_db_and_watch_slack(orc, mem):
     """learnt_cap 4 and one learned watch slot per literal force the DB-full restart
-    and the watch-overflow detach paths."""
-    print("[solve] tiny learned DB", flush=True)
+    and the watch-overflow detach paths (automatic and split clause memory)."""
+    print(f"[solve] tiny learned DB clause_mem={mem}", flush=True)
     b = W.random_3sat_batch(48, 60, 256, seed=5)
     with G.GpuBatch(b) as gb:
-        res = gb.solve(opts(learnt_cap=4, larena_cap=200, watch_slack=1))
+        res = gb.solve(opts(learnt_cap=4, larena_cap=200, watch_slack=1, clause_mem=mem))
         check_models(orc, gb, res, b)
     assert res["status"].tolist() == [truth(orc, b.instance(i)) for i in range(b.n_inst)]
 
