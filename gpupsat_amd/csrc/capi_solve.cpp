@@ -201,7 +201,15 @@ int gpsat_solve(gpsat_batch* b, const gpsat_solve_opts* opts, const int32_t* job
     if (o.clause_mem == 1) mode = lds_all <= lds_block_max ? 1 : 2;
     else if (o.clause_mem == 2) mode = 2;
     else if (o.clause_mem == 3) mode = lds_hyb <= lds_block_max ? 3 : 2;
-    else mode = lds_all <= std::min<size_t>(48 * 1024, lds_block_max) ? 1 : 2;
+    else {
+        // automatic: with few jobs the grid cannot fill the GPU and each job's latency is
+        // what counts, so the clause part goes to LDS as far as it fits; with many, LDS only
+        // while it leaves room for enough resident wavefronts
+        const bool few_jobs = n_jobs <= 4 * prop.multiProcessorCount;
+        if (lds_all <= (few_jobs ? lds_block_max : std::min<size_t>(48 * 1024, lds_block_max))) mode = 1;
+        else if (few_jobs && lds_hyb <= lds_block_max) mode = 3;
+        else mode = 2;
+    }
     const bool use_lds = mode == 1;
     const size_t lds_bytes = mode == 1 ? lds_all : mode == 3 ? lds_hyb : state_b;
     if (lds_bytes > lds_block_max) {

@@ -179,7 +179,9 @@ typedef struct gpsat_solve_opts {
                                 (0: automatic, up to 32 per CU as LDS allows) */
     int32_t clause_mem;      /* clause part of the job state: 0 automatic, 1 LDS, 2 HBM, 3 the
                                 original clauses' watch lists and pairs in LDS, the learned
-                                clauses' storage in HBM */
+                                clauses' storage in HBM.  Automatic: with at most 4 jobs per CU,
+                                LDS, else split, else HBM, whichever fits first; with more, LDS
+                                when it takes at most 48 KB per wavefront, else HBM */
     int32_t verify;          /* 1: re-check every SAT model with the clause-evaluation kernel */
     int32_t chunk;           /* jobs per queue grab (0: automatic); cubes of one instance in one
                                 chunk reuse its watch lists on the wavefront */
