@@ -240,6 +240,61 @@ __device__ int32_t walk_list(Ctx& J, int32_t fl, int32_t* W, int32_t base, int32
     return keep;
 }
 
+// walk_list for the original clauses of a batch whose clauses all have three literals, with
+// one divergent region (the watch moves) instead of a branch per test: every lane reads its
+// entry's clause, watch pair, the other watch's value and the third literal's value, and the
+// outcome (keep, move, unit, conflict) is computed from them.  A clause's third literal is the
+// sum of its literals minus its two watches, so no search loop either.
+__device__ int32_t walk_list3(Ctx& J, int32_t fl, int32_t base, int32_t cnt, int32_t& confl)
+{
+    const int lane = lane_id();
+    int32_t* W = J.Wo;
+    int32_t keep = 0;
+    for (int32_t i0 = 0; i0 < cnt; i0 += kWave) {
+        const int32_t i = i0 + lane;
+        const bool in_list = i < cnt;
+        const bool active = in_list && confl < 0;
+        // lanes past the end re-read the last entry (in bounds) and take no action
+        const int32_t c = W[base + min(i, cnt - 1)];
+        const int32_t w0 = J.wpair[2 * c], w1 = J.wpair[2 * c + 1];
+        const int32_t* cl = J.lits + 3 * c;
+        const uint32_t lsum = (uint32_t)cl[0] + (uint32_t)cl[1] + (uint32_t)cl[2];
+        const int32_t other = (w0 == fl) ? w1 : w0;
+        const int32_t q = (int32_t)(lsum - (uint32_t)w0 - (uint32_t)w1);
+        const int8_t vo = J.lv[other];
+        const int8_t vq = J.lv[q];
+        const bool watched = (w0 == fl) || (w1 == fl);   // else a stale entry: dropped
+        const bool open = active && watched && vo != 1;
+        const bool can_move = q != w0 && q != w1 && vq != -1;
+        const bool move = open && can_move;
+        const bool conf = open && !can_move && vo == -1;
+        const int32_t unit = (open && !can_move && vo == 0) ? other : -1;
+        const bool kept = in_list && !(active && (!watched || move));
+        if (move) {
+            const int32_t pos = atomicAdd(&J.wcnt_o[q], 1);
+            J.Wo[J.wstart[q] + pos] = c;
+            J.wpair[2 * c] = other;
+            J.wpair[2 * c + 1] = q;
+        }
+        const uint64_t km = ballot(kept);
+        if (kept) W[base + keep + prefix_count(km)] = c;
+        keep += popc64(km);
+        const uint64_t cm = ballot(conf);
+        if (cm && confl < 0) confl = uni(readlane(c, ctz64(cm)));
+        uint64_t um = ballot(unit >= 0);
+        while (um && confl < 0) {
+            const int j = ctz64(um);
+            um &= um - 1;
+            const int32_t u = uni(readlane(unit, j));
+            const int32_t rc = uni(readlane(c, j));
+            const int8_t val = J.lv[u];
+            if (val == -1) confl = rc;
+            else if (val == 0) assign(J, u, rc);
+        }
+    }
+    return keep;
+}
+
 __device__ int32_t propagate(Ctx& J)
 {
     int32_t confl = -1;
@@ -248,7 +303,8 @@ __device__ int32_t propagate(Ctx& J)
         J.qhead++;
         J.props++;
         const int32_t fl = p ^ 1;
-        int32_t k = walk_list<false>(J, fl, J.Wo, uni(J.wstart[fl]), uni(J.wcnt_o[fl]), confl);
+        const int32_t wb = uni(J.wstart[fl]), wc = uni(J.wcnt_o[fl]);
+        int32_t k = J.all3 ? walk_list3(J, fl, wb, wc, confl) : walk_list<false>(J, fl, J.Wo, wb, wc, confl);
         J.wcnt_o[fl] = k;
         k = walk_list<true>(J, fl, J.Wl, fl * J.slack, uni(J.wcnt_l[fl]), confl);
         J.wcnt_l[fl] = k;
