@@ -335,7 +335,9 @@ __device__ void backtrack(Ctx& J, int32_t b, bool save_phase)
 __device__ int32_t analyze(Ctx& J, int32_t confl, int32_t& bt, int32_t& lbd)
 {
     const int lane = lane_id();
-    int32_t nl = 1, pathc = 0, p = -1, idx = J.tsize - 1, c = confl;
+    // the loop state is wave-uniform; uni() keeps it in scalar registers, so the trail scan
+    // below is a scalar loop rather than an exec-masked one
+    int32_t nl = 1, pathc = 0, p = -1, idx = uni(J.tsize - 1), c = confl;
     for (;;) {
         if (c >= J.m) {
             const int32_t f = J.lmeta[4 * (c - J.m) + 2];
@@ -346,16 +348,18 @@ __device__ int32_t analyze(Ctx& J, int32_t confl, int32_t& bt, int32_t& lbd)
         clause_span(J, c, cl, len);
         for (int32_t k0 = 0; k0 < len; k0 += kWave) {
             const int32_t k = k0 + lane;
-            const int32_t q = k < len ? cl[k] : -1;
-            bool mark = false, cur = false;
-            if (q >= 0 && q != p) {
-                const int32_t v = q >> 1;
-                if (J.level[v] > 0 && atomicOr(&J.seen[v], 1) == 0) {
-                    mark = true;
-                    cur = J.level[v] == J.dl;
-                    J.act[v] += J.inc;
-                    J.bumps[v] += 1;
-                }
+            // every lane reads (lanes past the clause re-read its last literal and take no
+            // part), so the only divergent regions are the marking and its bookkeeping
+            const int32_t qr = cl[min(k, len - 1)];
+            const int32_t q = k < len ? qr : -1;
+            const int32_t v = qr >> 1;
+            const int32_t lvl = J.level[v];
+            bool mark = false;
+            if (q >= 0 && q != p && lvl > 0) mark = atomicOr(&J.seen[v], 1) == 0;
+            const bool cur = mark && lvl == J.dl;
+            if (mark) {
+                J.act[v] += J.inc;
+                J.bumps[v] += 1;
             }
             const uint64_t cmask = ballot(mark && cur);
             pathc += popc64(cmask);
@@ -367,14 +371,15 @@ __device__ int32_t analyze(Ctx& J, int32_t confl, int32_t& bt, int32_t& lbd)
         bool found = false;
         while (idx >= 0) {
             const int32_t t = idx - lane;
-            const bool hit = t >= 0 && J.seen[J.trail[t] >> 1] != 0;
+            const int32_t tl = J.trail[max(t, 0)];
+            const bool hit = t >= 0 && J.seen[tl >> 1] != 0;
             const uint64_t hm = ballot(hit);
             if (hm) {
-                idx -= ctz64(hm);
+                idx = uni(idx - ctz64(hm));
                 found = true;
                 break;
             }
-            idx -= kWave;
+            idx = uni(idx - kWave);
         }
         if (!found) return -1;   // no current-level literal: internal error, reported by the caller
         p = uni(J.trail[idx]);
