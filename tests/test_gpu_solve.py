@@ -257,15 +257,16 @@ def test_telemetry_phases(orc):
     assert (ph[js1[:, 1] > 0, 2] > 0).all()              # analysis time where there were conflicts
 
 
-def test_learned_clauses_are_implied(orc):
+@pytest.mark.parametrize("mem", [0, 2, 3])
+def test_learned_clauses_are_implied(orc, mem):
     """Soundness of learning: every learned clause a job still holds at its end, and every
     level-0 literal, is implied by the formula -- the formula plus the clause's negation as
     unit clauses is UNSAT for the oracle's CDCL.  A small learned DB (16 clauses) forces
     reductions; random two-literal clauses over the SAT instances show the check can fail."""
-    print("[solve] learned clauses implied: 40 x random 3-SAT n=50, learnt_cap 16", flush=True)
+    print(f"[solve] learned clauses implied: 40 x random 3-SAT n=50, learnt_cap 16, clause_mem={mem}", flush=True)
     b = W.random_3sat_batch(40, 50, 213, seed=77)
     with G.GpuBatch(b) as gb:
-        res = gb.solve(opts(learnt_cap=16), learned_cap=1 << 20)
+        res = gb.solve(opts(learnt_cap=16, clause_mem=mem), learned_cap=1 << 20)
     info = res["info"]
     assert info["learned_len"] == info["learned_need"] > 0
     blocks = res["learned"]
