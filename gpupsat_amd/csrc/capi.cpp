@@ -236,6 +236,15 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
     d.total_words = wbase;
     d.fast3 = fast3 && n_inst > 0;
     d.all3 = every3 && n_inst > 0;
+    // the register-resident kernel reads one 4-byte word per clause instead of three
+    std::vector<uint32_t> h_pk3;
+    if (d.fast3) {
+        h_pk3.resize(M);
+        for (int64_t c = 0; c < M; c++) {
+            const int32_t* q = lits + clause_off[c0 + c];
+            h_pk3[c] = (uint32_t)q[0] | ((uint32_t)q[1] << 10) | ((uint32_t)q[2] << 20);
+        }
+    }
     hipError_t e = hipGetDevice(&d.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
     if (e == hipSuccess && n_inst) e = hipMalloc(&d.meta, sizeof(InstMeta) * n_inst);
@@ -245,6 +254,8 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
     if (e == hipSuccess && L) e = hipMemcpy(d.lits, d.h_lits.data(), sizeof(int32_t) * L, hipMemcpyHostToDevice);
     if (e == hipSuccess && n_inst)
         e = hipMemcpy(d.coff, d.h_coff.data(), sizeof(int32_t) * (M + n_inst), hipMemcpyHostToDevice);
+    if (e == hipSuccess && d.fast3) e = hipMalloc(&d.pk3, sizeof(uint32_t) * M);
+    if (e == hipSuccess && d.fast3) e = hipMemcpy(d.pk3, h_pk3.data(), sizeof(uint32_t) * M, hipMemcpyHostToDevice);
     if (e != hipSuccess) {
         set_error(std::string("batch upload: ") + hipGetErrorString(e));
         gpsat_batch_destroy(b);
@@ -262,6 +273,7 @@ int gpsat_batch_destroy(gpsat_batch* b)
     (void)hipFree(d.meta);
     (void)hipFree(d.lits);
     (void)hipFree(d.coff);
+    (void)hipFree(d.pk3);
     if (d.stream) (void)hipStreamDestroy(d.stream);
     delete b;
     return GPSAT_OK;

@@ -273,10 +273,11 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
 
 // ---------------------------------------------------------------------------------------
 // Register-resident path for instances whose clauses all hold exactly three literals,
-// with n <= 256 (W <= 16 words per row) and m <= 1024 (<= 16 clause groups).  Every
-// global load of the instance (its A rows and its 3m literals) is issued before the
-// first one is used; each lane then keeps its 16 clauses as packed 10-bit literal
-// triples in VGPRs, so the BCP passes touch LDS only.  With BCP, the input rows are copied
+// with n <= 256 (W <= 16 words per row) and m <= 1024 (<= 16 clause groups).  The clauses
+// come from the batch's packed clause words (three 10-bit literal codes per 4-byte word).
+// Every global load of the instance (its A rows and its m clause words) is issued before
+// the first one is used; each lane then keeps its 16 clause words in VGPRs, so the BCP
+// passes touch LDS only.  With BCP, the input rows are copied
 // to the output as they arrive; only rows that gain implications are written again.
 //
 // LDS holds the literal masks twice: slot l (16 bytes) = {T(l), T(l ^ 1)}, the true and
@@ -304,7 +305,7 @@ __device__ __forceinline__ uint64_t u64(uint32_t lo, uint32_t hi) { return (uint
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
 template <bool DO_EVAL, bool DO_BCP>
-__global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ meta, const int32_t* __restrict__ lits,
+__global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ meta, const uint32_t* __restrict__ pk3,
                                                     const uint32_t* __restrict__ rows, int A, int32_t* __restrict__ counts,
                                                     uint32_t* __restrict__ out_rows, int32_t* __restrict__ conflict,
                                                     int32_t* __restrict__ props, int32_t* __restrict__ passes_out,
@@ -322,7 +323,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     // addresses of all loads are known before any of them is waited for
     const int W = __builtin_amdgcn_readfirstlane(words_per_row(mi.n));
     const int m = __builtin_amdgcn_readfirstlane(mi.m);
-    const int64_t lit_base = uni64(mi.lit_base);
+    const int64_t clause_base = uni64(mi.coff_base - inst);
     const int64_t word_base = uni64(mi.word_base);
     const int G = (m + kWave - 1) / kWave;
     // slot 32*W stays zero: lanes past m evaluate a clause of three such literals, which
@@ -333,9 +334,9 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     const u32x4* dup4 = reinterpret_cast<const u32x4*>(dup);
     uint64_t* dup64 = reinterpret_cast<uint64_t*>(dup);
     const int64_t row0 = (int64_t)A * word_base;
-    const int32_t* ilits = lits + lit_base;
+    const uint32_t* ipk = pk3 + clause_base;
 
-    // rows: coalesced copy of the A*W words into LDS; literals: straight into VGPRs.
+    // rows: coalesced copy of the A*W words into LDS; clause words: straight into VGPRs.
     // Every load sits in a scalar-guarded block with no use inside it (lanes past the end
     // re-read the last word), so all of them are in flight together.
     const uint32_t* src = rows + row0;
@@ -346,20 +347,15 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
         rbuf[u] = 0u;
         if (u * kWave < nw) rbuf[u] = __builtin_nontemporal_load(src + u * kWave + min(lane, nw - 1 - u * kWave));
     }
-    // literal loads: one scalar-guarded block per group with no use inside it, so all
-    // 3G loads are in flight together (lanes past m re-read clause m-1, which stays in
-    // bounds and is discarded when the triples are packed below)
-    uint32_t la[kFastG], lb[kFastG], lc[kFastG];
+    // clause words: one scalar-guarded block per group with no use inside it, so all G
+    // loads are in flight together (lanes past m re-read clause m-1, which stays in bounds
+    // and is replaced by the zero triple below)
+    uint32_t pk[kFastG];
     const int mc = m > 0 ? m - 1 : 0;
 #pragma unroll
     for (int g = 0; g < kFastG; g++) {
-        la[g] = lb[g] = lc[g] = 0u;
-        if (g < G) {
-            const int32_t* p = ilits + 3 * g * kWave + 3 * min(lane, mc - g * kWave);
-            la[g] = (uint32_t)__builtin_nontemporal_load(p);
-            lb[g] = (uint32_t)__builtin_nontemporal_load(p + 1);
-            lc[g] = (uint32_t)__builtin_nontemporal_load(p + 2);
-        }
+        pk[g] = 0u;
+        if (g < G) pk[g] = __builtin_nontemporal_load(ipk + g * kWave + min(lane, mc - g * kWave));
     }
     // a conflicting row's output is its input: every output row starts as a copy (the
     // rows that gain implications are rewritten at the end).  Word offsets are split
@@ -382,6 +378,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
         in[j] = (j < W && lane < A) ? dup[lane * W + j] : 0u;
         in_pop += __builtin_popcount(in[j]);
     }
+    // the count is formed here: sunk to its use at the end, it would keep all W words live
+    asm volatile("" : "+v"(in_pop));
     wave_sync();
     const Transposer tp(lane);
     const int half = lane >> 5, i = lane & 31;
@@ -395,12 +393,10 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
             dup[4 * (l ^ 1) + 2 + half] = x;
         }
     }
-    // the packed triples are formed here, once: the empty asm keeps the compiler from
-    // sinking the packing into the groups, which would hold all 3G unpacked literals live
-    uint32_t pk[kFastG];
+    // lanes past m take the zero triple; the empty asm keeps the select out of the groups
 #pragma unroll
     for (int g = 0; g < kFastG; g++) {
-        pk[g] = (lane < m - g * kWave) ? (la[g] | (lb[g] << 10) | (lc[g] << 20)) : zpk;
+        pk[g] = (lane < m - g * kWave) ? pk[g] : zpk;
         asm volatile("" : "+v"(pk[g]));
     }
     wave_sync();
@@ -834,13 +830,13 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
         dim3 grid((b.n_inst + wpb - 1) / wpb), block(kWave * wpb);
         const size_t lds = (size_t)wpb * lds_words * 4;
         if (do_eval && do_bcp)
-            hipLaunchKernelGGL((k_eval_bcp3<true, true>), grid, block, lds, st, b.meta, b.lits, rows, A, counts, out_rows,
+            hipLaunchKernelGGL((k_eval_bcp3<true, true>), grid, block, lds, st, b.meta, b.pk3, rows, A, counts, out_rows,
                                conflict, props, passes, b.n_inst, lds_words);
         else if (do_eval)
-            hipLaunchKernelGGL((k_eval_bcp3<true, false>), grid, block, lds, st, b.meta, b.lits, rows, A, counts,
+            hipLaunchKernelGGL((k_eval_bcp3<true, false>), grid, block, lds, st, b.meta, b.pk3, rows, A, counts,
                                out_rows, conflict, props, passes, b.n_inst, lds_words);
         else
-            hipLaunchKernelGGL((k_eval_bcp3<false, true>), grid, block, lds, st, b.meta, b.lits, rows, A, counts,
+            hipLaunchKernelGGL((k_eval_bcp3<false, true>), grid, block, lds, st, b.meta, b.pk3, rows, A, counts,
                                out_rows, conflict, props, passes, b.n_inst, lds_words);
         return hipGetLastError();
     }

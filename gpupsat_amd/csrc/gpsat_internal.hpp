@@ -23,6 +23,9 @@ struct DeviceBatch {
     InstMeta* meta = nullptr;     // device [n_inst]
     int32_t* lits = nullptr;      // device [n_lits]
     int32_t* coff = nullptr;      // device [n_clauses + n_inst], per-instance relative offsets
+    // device [n_clauses] when fast3: clause c's three literal codes in bits 0-9, 10-19 and
+    // 20-29 (codes are below 2n <= 512); instance i's clauses start at meta.coff_base - i
+    uint32_t* pk3 = nullptr;
     hipStream_t stream = nullptr;
     int device = 0;
     // host mirrors used by the solver set-up and the job chooser
