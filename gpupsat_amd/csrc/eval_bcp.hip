@@ -90,17 +90,33 @@ __device__ __forceinline__ ClauseMasks clause_masks(const int32_t* __restrict__ 
 
 // Accumulate per-assignment totals of one 64-clause group.  b0 = SAT|UNIT, b1 = UNSAT|UNIT
 // per clause (lane); after the per-half transposes lane r holds, for assignment r
-// (low word) and 32+r (high word), the statuses of the 32 clauses of its half.
+// (low word) and 32+r (high word), the statuses of the 32 clauses of its half.  The
+// accumulators count b0, b1 and UNIT = b0 & b1; finish_counts turns them into
+// #SAT = |b0| - #UNIT and #UNSAT = |b1| - #UNIT.
 __device__ __forceinline__ void count_group(uint64_t b0, uint64_t b1, const Transposer& tp, uint32_t (&acc)[2][3])
 {
     const uint32_t w0[2] = {tp((uint32_t)b0), tp((uint32_t)(b0 >> 32))};
     const uint32_t w1[2] = {tp((uint32_t)b1), tp((uint32_t)(b1 >> 32))};
 #pragma unroll
     for (int h = 0; h < 2; h++) {
-        acc[h][0] += __builtin_popcount(w0[h] & ~w1[h]);   // SAT
-        acc[h][1] += __builtin_popcount(w1[h] & ~w0[h]);   // UNSAT
-        acc[h][2] += __builtin_popcount(w0[h] & w1[h]);    // UNIT
+        acc[h][0] += __builtin_popcount(w0[h]);
+        acc[h][1] += __builtin_popcount(w1[h]);
+        acc[h][2] += __builtin_popcount(w0[h] & w1[h]);
     }
+}
+
+// Per-assignment #SAT, #UNSAT, #UNIT from count_group's accumulators: assignment a sums
+// the low words of lanes a and a^32 (a < 32) or the high words (a >= 32).
+__device__ __forceinline__ void finish_counts(const uint32_t (&acc)[2][3], int lane, uint32_t (&tot)[3])
+{
+#pragma unroll
+    for (int s = 0; s < 3; s++) {
+        const uint32_t lo = acc[0][s] + shfl_xor32(acc[0][s]);
+        const uint32_t hi = acc[1][s] + shfl_xor32(acc[1][s]);
+        tot[s] = (lane < 32) ? lo : hi;
+    }
+    tot[0] -= tot[2];
+    tot[1] -= tot[2];
 }
 
 // ---------------------------------------------------------------------------------------
@@ -163,14 +179,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
             }
             count_group(b0, b1, tp, acc);
         }
-        // assignment a: low word of lanes a and a^32 (a < 32) or high words (a >= 32)
         uint32_t tot[3];
-#pragma unroll
-        for (int s = 0; s < 3; s++) {
-            const uint32_t lo = acc[0][s] + shfl_xor32(acc[0][s]);
-            const uint32_t hi = acc[1][s] + shfl_xor32(acc[1][s]);
-            tot[s] = (lane < 32) ? lo : hi;
-        }
+        finish_counts(acc, lane, tot);
         if (lane < A) {
             int32_t* o = counts + ((int64_t)inst * A + lane) * 3;
             o[0] = (int32_t)tot[0];
@@ -299,6 +309,28 @@ constexpr int kFastG = kFast3MaxGroups;
 
 __device__ __forceinline__ uint64_t u64(uint32_t lo, uint32_t hi) { return (uint64_t)lo | ((uint64_t)hi << 32); }
 
+// Symmetric 3-input bit functions as one v_bitop3_b32 each (the truth tables are invariant
+// under any order of the three sources): at least two of a, b, c set; exactly two set;
+// all three set.
+__device__ __forceinline__ uint32_t maj3(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t d;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0xe8" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+__device__ __forceinline__ uint32_t two3(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t d;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x68" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+__device__ __forceinline__ uint32_t and3(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t d;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x80" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+
 // a native 4-word vector: 16-byte LDS reads of it stay single ds_read_b128 instructions
 // (the HIP uint4 struct is split into 8-byte halves by the compiler when its members are
 // consumed as 64-bit pairs)
@@ -412,14 +444,13 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
             b = dup4[(p >> 10) & 1023u];
             c = dup4[p >> 20];
         };
+        // per clause: SAT|UNIT = T | (exactly two literals false), UNSAT|UNIT = (at least
+        // two false) & ~T, two operations per 32-bit word each
         const auto eval_group = [&](const u32x4& qa, const u32x4& qb, const u32x4& qc) {
-            const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
-            const uint64_t T = (u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y));
-            const uint64_t all = Fa & Fb & Fc;
-            const uint64_t unit = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T;
-            const uint64_t unsat = all & ~T;
-            unsat_any |= unsat;
-            count_group(T | unit, unsat | unit, tp, acc);
+            const uint32_t Tl = qa.x | qb.x | qc.x, Th = qa.y | qb.y | qc.y;
+            const uint32_t b0l = Tl | two3(qa.z, qb.z, qc.z), b0h = Th | two3(qa.w, qb.w, qc.w);
+            const uint32_t b1l = maj3(qa.z, qb.z, qc.z) & ~Tl, b1h = maj3(qa.w, qb.w, qc.w) & ~Th;
+            count_group(u64(b0l, b0h), u64(b1l, b1h), tp, acc);
         };
         u32x4 xa, xb, xc, ya, yb, yc;
         fetch(pk[0], xa, xb, xc);
@@ -435,18 +466,14 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
             }
         }
         uint32_t tot[3];
-#pragma unroll
-        for (int s = 0; s < 3; s++) {
-            const uint32_t lo = acc[0][s] + shfl_xor32(acc[0][s]);
-            const uint32_t hi = acc[1][s] + shfl_xor32(acc[1][s]);
-            tot[s] = (lane < 32) ? lo : hi;
-        }
+        finish_counts(acc, lane, tot);
         if (lane < A) {
             int32_t* o = counts + ((int64_t)inst * A + lane) * 3;
             o[0] = (int32_t)tot[0];
             o[1] = (int32_t)tot[1];
             o[2] = (int32_t)tot[2];
         }
+        unsat_any = ballot(lane < A && tot[1] != 0u);
     }
 
     if (DO_BCP) {
@@ -574,9 +601,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                 const auto group = [&](uint32_t p, const uint2& qa, const uint2& qb, const uint2& qc) {
                     const uint32_t Fa = qa.y, Fb = qb.y, Fc = qc.y;
                     const uint32_t T = qa.x | qb.x | qc.x;
-                    const uint32_t all = Fa & Fb & Fc;
-                    const uint32_t newconf = all & ~T & ~fz;
-                    const uint32_t U = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T & ~fz;
+                    const uint32_t newconf = and3(Fa, Fb, Fc) & ~(T | fz);
+                    const uint32_t U = two3(Fa, Fb, Fc) & ~(T | fz);
                     if (U) {
                         const uint32_t la = p & 1023u, lb = (p >> 10) & 1023u, lc = p >> 20;
                         const uint32_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
@@ -669,12 +695,11 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                     c = dup4[p >> 20];
                 };
                 const auto group = [&](uint32_t p, const u32x4& qa, const u32x4& qb, const u32x4& qc) {
-                    const uint64_t valid = ~frozen;
                     const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
                     const uint64_t T = u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y);
-                    const uint64_t all = Fa & Fb & Fc;
-                    const uint64_t newconf = all & ~T & valid;
-                    const uint64_t U = ((Fa & Fb) | (Fc & (Fa ^ Fb))) & ~all & ~T & valid;
+                    const uint64_t open = ~(T | frozen);
+                    const uint64_t newconf = u64(and3(qa.z, qb.z, qc.z), and3(qa.w, qb.w, qc.w)) & open;
+                    const uint64_t U = u64(two3(qa.z, qb.z, qc.z), two3(qa.w, qb.w, qc.w)) & open;
                     if (U) {
                         const uint32_t la = p & 1023u, lb = (p >> 10) & 1023u, lc = p >> 20;
                         const uint64_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
