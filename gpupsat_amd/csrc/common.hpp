@@ -104,10 +104,12 @@ __device__ __forceinline__ void wave_sync()
 // (no LDS): quads, then rows of 16, then the two rows of each half, then the halves.
 __device__ __forceinline__ uint32_t wave_or32(uint32_t x)
 {
-    x |= __builtin_amdgcn_mov_dpp(x, 0xB1, 0xF, 0xF, false);    // quad_perm [1,0,3,2]
-    x |= __builtin_amdgcn_mov_dpp(x, 0x4E, 0xF, 0xF, false);    // quad_perm [2,3,0,1]
-    x |= __builtin_amdgcn_mov_dpp(x, 0x124, 0xF, 0xF, false);   // row_ror:4
-    x |= __builtin_amdgcn_mov_dpp(x, 0x128, 0xF, 0xF, false);   // row_ror:8
+    // bound_ctrl with a zero old value (the OR identity) lets the compiler fold each move
+    // into its OR as one v_or_b32 with a DPP source
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0xB1, 0xF, 0xF, true);    // quad_perm [1,0,3,2]
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0x4E, 0xF, 0xF, true);    // quad_perm [2,3,0,1]
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0x124, 0xF, 0xF, true);   // row_ror:4
+    x |= __builtin_amdgcn_update_dpp(0u, x, 0x128, 0xF, 0xF, true);   // row_ror:8
     const auto r = __builtin_amdgcn_permlane16_swap(x, x, false, false);
     x = r[0] | r[1];
     const auto q = __builtin_amdgcn_permlane32_swap(x, x, false, false);

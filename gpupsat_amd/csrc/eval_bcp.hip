@@ -296,7 +296,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
 // l ^ 1.  The first 64*W words double as the row staging area; slot 32*W is all zero.
 // With at most 32 open rows, BCP runs on a compacted table of 8-byte slots
 // {T(l), T(l ^ 1)} over the open rows, at words [64*W, 128*W + 2).
-// LDS: (128*W + 4) words.
+// LDS: (128*W + 8) words, so every wavefront's region starts 32-byte aligned (the BCP passes
+// find a slot's partner l ^ 1 by flipping one address bit).
 __device__ __forceinline__ int64_t uni64(int64_t x)
 {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
@@ -336,6 +337,35 @@ __device__ __forceinline__ uint32_t and3(uint32_t a, uint32_t b, uint32_t c)
 // consumed as 64-bit pairs)
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 
+// LDS accesses by byte address: a BCP pass forms each clause literal's slot address once and
+// uses it for the slot read and for both implication writes.  The partner slot l ^ 1 is the
+// address with bit 3 flipped in a table of 8-byte slots starting 16-byte aligned, and with
+// bit 4 flipped in a table of 16-byte slots starting 32-byte aligned.
+typedef __attribute__((address_space(3))) uint32_t lds_u32;
+typedef __attribute__((address_space(3))) unsigned long long lds_u64;
+typedef uint32_t u32x2 __attribute__((ext_vector_type(2)));
+typedef __attribute__((address_space(3))) u32x2 lds_u32x2;
+typedef __attribute__((address_space(3))) u32x4 lds_u32x4;
+__device__ __forceinline__ uint32_t lds_addr(const void* p)
+{
+    return (uint32_t)(uintptr_t)(const __attribute__((address_space(3))) void*)p;
+}
+__device__ __forceinline__ uint2 lds_read8(uint32_t a)
+{
+    const u32x2 v = *(const lds_u32x2*)(uintptr_t)a;
+    return make_uint2(v.x, v.y);
+}
+__device__ __forceinline__ u32x4 lds_read16(uint32_t a) { return *(const lds_u32x4*)(uintptr_t)a; }
+__device__ __forceinline__ void lds_or32(uint32_t a, uint32_t v)
+{
+    __hip_atomic_fetch_or((lds_u32*)(uintptr_t)a, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ void lds_or64(uint32_t a, uint64_t v)
+{
+    __hip_atomic_fetch_or((lds_u64*)(uintptr_t)a, (unsigned long long)v, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
 template <bool DO_EVAL, bool DO_BCP>
 __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ meta, const uint32_t* __restrict__ pk3,
                                                     const uint32_t* __restrict__ rows, int A, int32_t* __restrict__ counts,
@@ -343,7 +373,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                                                     int32_t* __restrict__ props, int32_t* __restrict__ passes_out,
                                                     int n_inst, int lds_words)
 {
-    extern __shared__ __align__(16) uint32_t lds_all[];
+    extern __shared__ __align__(32) uint32_t lds_all[];
     // the wavefront's slot is uniform: its metadata then comes in with scalar loads
     const int wslot = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const int inst = blockIdx.x * (blockDim.x / kWave) + wslot;
@@ -364,7 +394,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     const uint32_t zpk = zslot | (zslot << 10) | (zslot << 20);
     uint32_t* dup = smem;                        // 32*W slots of 4 words, then the zero slot
     const u32x4* dup4 = reinterpret_cast<const u32x4*>(dup);
-    uint64_t* dup64 = reinterpret_cast<uint64_t*>(dup);
+    const uint32_t dupb = lds_addr(dup);         // its LDS byte address (32-byte aligned)
     const int64_t row0 = (int64_t)A * word_base;
     const uint32_t* ipk = pk3 + clause_base;
 
@@ -503,6 +533,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
             const int k_me = __builtin_popcountll(act & ((1ull << lane) - 1ull));
             uint32_t* ct = dup + 64 * W;            // slot l = {T(l), T(l ^ 1)}, then the zero slot
             const uint2* ct2 = reinterpret_cast<const uint2*>(ct);
+            const uint32_t ctb = lds_addr(ct);      // 32-byte aligned: 256*W bytes past dup
             int a_k = 0;                            // lane k < n_act: row index of open row k
             if (n_act <= 8) {
                 // the open rows' bits in the 64-bit table are still the input's (nothing has
@@ -588,35 +619,37 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                 asm volatile("" : "+s"(Gp));
                 uint32_t chg = 0;       // any implication written by this lane
                 // same two-buffer pipeline as the evaluation; the empty asm keeps each
-                // group's unpacked literals inside the pass
-                const auto fetch = [&](uint32_t& p, int g, uint2& a, uint2& b, uint2& c) {
-                    p = pk[g];
+                // group's unpacked literals inside the pass.  s[] = the slot addresses.
+                const auto fetch = [&](uint32_t (&s)[3], int g, uint2& a, uint2& b, uint2& c) {
+                    uint32_t p = pk[g];
                     asm volatile("" : "+v"(p));
-                    a = ct2[p & 1023u];
-                    b = ct2[(p >> 10) & 1023u];
-                    c = ct2[p >> 20];
+                    s[0] = ctb + 8u * (p & 1023u);
+                    s[1] = ctb + 8u * ((p >> 10) & 1023u);
+                    s[2] = ctb + 8u * (p >> 20);
+                    a = lds_read8(s[0]);
+                    b = lds_read8(s[1]);
+                    c = lds_read8(s[2]);
                 };
                 // one group: implications of its unit clauses, new conflicts; true once
                 // every open row is in conflict
-                const auto group = [&](uint32_t p, const uint2& qa, const uint2& qb, const uint2& qc) {
+                const auto group = [&](const uint32_t (&s)[3], const uint2& qa, const uint2& qb, const uint2& qc) {
                     const uint32_t Fa = qa.y, Fb = qb.y, Fc = qc.y;
                     const uint32_t T = qa.x | qb.x | qc.x;
                     const uint32_t newconf = and3(Fa, Fb, Fc) & ~(T | fz);
                     const uint32_t U = two3(Fa, Fb, Fc) & ~(T | fz);
                     if (U) {
-                        const uint32_t la = p & 1023u, lb = (p >> 10) & 1023u, lc = p >> 20;
                         const uint32_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
                         if (Ua) {
-                            atomicOr(&ct[2 * la], Ua);
-                            atomicOr(&ct[2 * (la ^ 1u) + 1], Ua);
+                            lds_or32(s[0], Ua);
+                            lds_or32((s[0] ^ 8u) + 4u, Ua);
                         }
                         if (Ub) {
-                            atomicOr(&ct[2 * lb], Ub);
-                            atomicOr(&ct[2 * (lb ^ 1u) + 1], Ub);
+                            lds_or32(s[1], Ub);
+                            lds_or32((s[1] ^ 8u) + 4u, Ub);
                         }
                         if (Uc) {
-                            atomicOr(&ct[2 * lc], Uc);
-                            atomicOr(&ct[2 * (lc ^ 1u) + 1], Uc);
+                            lds_or32(s[2], Uc);
+                            lds_or32((s[2] ^ 8u) + 4u, Uc);
                         }
                     }
                     chg |= U;
@@ -626,7 +659,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                     }
                     return false;
                 };
-                uint32_t px, py;
+                uint32_t px[3], py[3];
                 uint2 xa, xb, xc, ya, yb, yc;
                 fetch(px, 0, xa, xb, xc);
 #pragma unroll
@@ -687,33 +720,35 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                 // loop ends only after a pass without a single write, or with every row in
                 // conflict).  The empty asm keeps the unpacked literals inside the pass
                 // instead of hoisted into 48 live registers.
-                const auto fetch = [&](uint32_t& p, int g, u32x4& a, u32x4& b, u32x4& c) {
-                    p = pk[g];
+                const auto fetch = [&](uint32_t (&s)[3], int g, u32x4& a, u32x4& b, u32x4& c) {
+                    uint32_t p = pk[g];
                     asm volatile("" : "+v"(p));
-                    a = dup4[p & 1023u];
-                    b = dup4[(p >> 10) & 1023u];
-                    c = dup4[p >> 20];
+                    s[0] = dupb + 16u * (p & 1023u);
+                    s[1] = dupb + 16u * ((p >> 10) & 1023u);
+                    s[2] = dupb + 16u * (p >> 20);
+                    a = lds_read16(s[0]);
+                    b = lds_read16(s[1]);
+                    c = lds_read16(s[2]);
                 };
-                const auto group = [&](uint32_t p, const u32x4& qa, const u32x4& qb, const u32x4& qc) {
+                const auto group = [&](const uint32_t (&s)[3], const u32x4& qa, const u32x4& qb, const u32x4& qc) {
                     const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
                     const uint64_t T = u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y);
                     const uint64_t open = ~(T | frozen);
                     const uint64_t newconf = u64(and3(qa.z, qb.z, qc.z), and3(qa.w, qb.w, qc.w)) & open;
                     const uint64_t U = u64(two3(qa.z, qb.z, qc.z), two3(qa.w, qb.w, qc.w)) & open;
                     if (U) {
-                        const uint32_t la = p & 1023u, lb = (p >> 10) & 1023u, lc = p >> 20;
                         const uint64_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
                         if (Ua) {
-                            atomicOr((unsigned long long*)&dup64[2 * la], (unsigned long long)Ua);
-                            atomicOr((unsigned long long*)&dup64[2 * (la ^ 1u) + 1], (unsigned long long)Ua);
+                            lds_or64(s[0], Ua);
+                            lds_or64((s[0] ^ 16u) + 8u, Ua);
                         }
                         if (Ub) {
-                            atomicOr((unsigned long long*)&dup64[2 * lb], (unsigned long long)Ub);
-                            atomicOr((unsigned long long*)&dup64[2 * (lb ^ 1u) + 1], (unsigned long long)Ub);
+                            lds_or64(s[1], Ub);
+                            lds_or64((s[1] ^ 16u) + 8u, Ub);
                         }
                         if (Uc) {
-                            atomicOr((unsigned long long*)&dup64[2 * lc], (unsigned long long)Uc);
-                            atomicOr((unsigned long long*)&dup64[2 * (lc ^ 1u) + 1], (unsigned long long)Uc);
+                            lds_or64(s[2], Uc);
+                            lds_or64((s[2] ^ 16u) + 8u, Uc);
                         }
                     }
                     chg |= (uint32_t)U | (uint32_t)(U >> 32);
@@ -723,7 +758,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                     }
                     return false;
                 };
-                uint32_t px, py;
+                uint32_t px[3], py[3];
                 u32x4 xa, xb, xc, ya, yb, yc;
                 fetch(px, 0, xa, xb, xc);
 #pragma unroll
@@ -850,7 +885,7 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
 {
     if (b.n_inst == 0) return hipSuccess;
     if (b.fast3 && tuning().eval_fast3) {
-        const int lds_words = 128 * words_per_row(b.max_n) + 4;
+        const int lds_words = 128 * words_per_row(b.max_n) + 8;   // 32-byte multiple
         const int wpb = waves_per_block(lds_words);
         dim3 grid((b.n_inst + wpb - 1) / wpb), block(kWave * wpb);
         const size_t lds = (size_t)wpb * lds_words * 4;
