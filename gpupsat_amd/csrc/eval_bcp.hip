@@ -520,22 +520,173 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
         }
         uint64_t frozen = frozen0;
         int pass = 0;
-        bool my_conflict;
+        bool my_conflict = (frozen >> lane) & 1ull;
         int cnt = 0;            // literals in the closure of an open row
-        const uint64_t act = amask & ~frozen;
-        const int n_act = __builtin_popcountll(act);
+        uint64_t act = amask & ~frozen;
+        int n_act = __builtin_popcountll(act);
+        // More than 32 open rows: passes over the 64-bit table.  A row that gains no
+        // implication during a whole pass is at its fixpoint (every clause was evaluated
+        // against its final state).  When, from the second pass on, 8 to 32 rows still gain
+        // implications, the others' closures are staged for output and the passes go on over a
+        // compacted table of the rows still open (a handoff).
+        bool handoff = false;
+        int a_k = 0;            // lane k < n_act: row index of open row k
+        if (n_act > 32) {
+            bool changed = true;
+            while (changed) {
+                pass++;
+                // per-pass copy of the group count that the compiler cannot hoist: the
+                // group guards are then re-tested with scalar compares instead of being
+                // kept (and spilled) as masks across the passes
+                int Gp = G;
+                asm volatile("" : "+s"(Gp));
+                uint64_t chg = 0;       // rows in which this lane's clauses implied a literal
+                // the reads of group g+1 are issued before group g applies its implications,
+                // so they may miss them until the next pass (the fixpoint is unchanged: the
+                // loop ends only after a pass without a single write, or with every row in
+                // conflict).  The empty asm keeps the unpacked literals inside the pass
+                // instead of hoisted into 48 live registers.
+                const auto fetch = [&](uint32_t (&s)[3], int g, u32x4& a, u32x4& b, u32x4& c) {
+                    uint32_t p = pk[g];
+                    asm volatile("" : "+v"(p));
+                    s[0] = dupb + 16u * (p & 1023u);
+                    s[1] = dupb + 16u * ((p >> 10) & 1023u);
+                    s[2] = dupb + 16u * (p >> 20);
+                    a = lds_read16(s[0]);
+                    b = lds_read16(s[1]);
+                    c = lds_read16(s[2]);
+                };
+                const auto group = [&](const uint32_t (&s)[3], const u32x4& qa, const u32x4& qb, const u32x4& qc) {
+                    const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
+                    const uint64_t T = u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y);
+                    const uint64_t open = ~(T | frozen);
+                    const uint64_t newconf = u64(and3(qa.z, qb.z, qc.z), and3(qa.w, qb.w, qc.w)) & open;
+                    const uint64_t U = u64(two3(qa.z, qb.z, qc.z), two3(qa.w, qb.w, qc.w)) & open;
+                    if (U) {
+                        const uint64_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
+                        if (Ua) {
+                            lds_or64(s[0], Ua);
+                            lds_or64((s[0] ^ 16u) + 8u, Ua);
+                        }
+                        if (Ub) {
+                            lds_or64(s[1], Ub);
+                            lds_or64((s[1] ^ 16u) + 8u, Ub);
+                        }
+                        if (Uc) {
+                            lds_or64(s[2], Uc);
+                            lds_or64((s[2] ^ 16u) + 8u, Uc);
+                        }
+                    }
+                    chg |= U;
+                    if (ballot(newconf != 0)) {
+                        frozen |= wave_or64(newconf);
+                        return (frozen & amask) == amask;
+                    }
+                    return false;
+                };
+                uint32_t px[3], py[3];
+                u32x4 xa, xb, xc, ya, yb, yc;
+                fetch(px, 0, xa, xb, xc);
+#pragma unroll
+                for (int g = 0; g < kFastG; g += 2) {
+                    if (g < Gp) {
+                        if (g + 1 < kFastG) fetch(py, g + 1, ya, yb, yc);
+                        if (group(px, xa, xb, xc)) break;
+                    }
+                    if (g + 1 < Gp) {
+                        if (g + 2 < kFastG) fetch(px, g + 2, xa, xb, xc);
+                        if (group(py, ya, yb, yc)) break;
+                    }
+                }
+                const uint64_t live = wave_or64(chg) & amask & ~frozen;   // rows still moving
+                changed = live != 0ull;
+                wave_sync();
+                // the handoff adds latency to the wavefront (a chain of LDS round trips), so it
+                // waits for the second pass and for at least 8 rows still moving: with fewer,
+                // or after the first pass at low trail density, the next pass is usually the
+                // last and the handoff would lengthen the launch's tail
+                const int n_live = __builtin_popcountll(live);
+                if (pass >= 2 && n_live >= 8 && n_live <= 32) {
+                    handoff = true;
+                    act = live;
+                    changed = false;
+                }
+            }
+            // complementary implications are conflicts too (nothing to look for once every
+            // row is in conflict)
+            if ((frozen & amask) != amask) {
+                uint64_t cm = 0;
+                for (int v = lane; v < 16 * W; v += kWave) {
+                    const u32x4 q = dup4[2 * v];
+                    cm |= u64(q.x, q.y) & u64(q.z, q.w);
+                }
+                frozen |= wave_or64(cm);
+            }
+            my_conflict = (frozen >> lane) & 1ull;
+            act &= ~frozen;
+            n_act = handoff ? __builtin_popcountll(act) : 0;
+            if ((frozen & amask) != amask) {
+                uint32_t hx[kFastW];    // lane a: word j of row a
+#pragma unroll
+                for (int j = 0; j < kFastW; j++) {
+                    hx[j] = 0u;
+                    if (j < W) {
+                        hx[j] = tp(dup[4 * (32 * j + i) + half]);   // lane a: word j of row a
+                        cnt += __builtin_popcount(hx[j]);
+                    }
+                }
+                wave_sync();
+                // the rows at their fixpoint are staged (the 64-bit table is dead from here on)
+                const bool still_open = handoff && ((act >> lane) & 1ull);
+                if (lane < A && !my_conflict && !still_open) {
+#pragma unroll
+                    for (int j = 0; j < kFastW; j++)
+                        if (j < W) dup[lane * W + j] = hx[j];
+                }
+                if (n_act > 0) {
+                    // lane k < n_act takes still-open row k: its index through a map in the
+                    // upper half of the dead table, its words from the row's lane; then the
+                    // compacted table is built here, as in the passes' own set-up below
+                    uint32_t* ct = dup + 64 * W;
+                    if ((act >> lane) & 1ull) ct[__builtin_popcountll(act & ((1ull << lane) - 1ull))] = (uint32_t)lane;
+                    wave_sync();
+                    a_k = lane < n_act ? (int)ct[lane] : 0;
+                    wave_sync();
+#pragma unroll
+                    for (int j = 0; j < kFastW; j++) {
+                        const uint32_t w = (uint32_t)__builtin_amdgcn_ds_bpermute(a_k << 2, (int)hx[j]);
+                        hx[j] = lane < n_act ? w : 0u;
+                    }
+                    if (lane < 2) ct[64 * W + lane] = 0u;
+#pragma unroll
+                    for (int j = 0; j < kFastW; j++)
+                        if (j < W) hx[j] = tp(hx[j]);      // lane l < 32: literal 32j+l
+                    if (lane < 32) {
+#pragma unroll
+                        for (int j = 0; j < kFastW; j++) {
+                            if (j < W) {
+                                const int l = 32 * j + lane;
+                                ct[2 * l] = hx[j];
+                                ct[2 * (l ^ 1) + 1] = hx[j];
+                            }
+                        }
+                    }
+                }
+            }
+        }
         if (n_act > 0 && n_act <= 32) {
             // Few rows still open (the usual case after the evaluation already froze every
-            // row with a falsified clause): their bits are gathered into 32-bit masks, so
-            // the passes read 8 bytes per literal and run 32-bit logic.  Rows are
-            // independent, so the closure of each row is unchanged.
+            // row with a falsified clause, or after a handoff): their bits are gathered into
+            // 32-bit masks, so the passes read 8 bytes per literal and run 32-bit logic.
+            // Rows are independent, so the closure of each row is unchanged.
             const bool me_act = (act >> lane) & 1ull;
             const int k_me = __builtin_popcountll(act & ((1ull << lane) - 1ull));
             uint32_t* ct = dup + 64 * W;            // slot l = {T(l), T(l ^ 1)}, then the zero slot
             const uint2* ct2 = reinterpret_cast<const uint2*>(ct);
             const uint32_t ctb = lds_addr(ct);      // 32-byte aligned: 256*W bytes past dup
-            int a_k = 0;                            // lane k < n_act: row index of open row k
-            if (n_act <= 8) {
+            if (handoff) {
+                // the compacted table was built at the handoff
+            } else if (n_act <= 8) {
                 // the open rows' bits in the 64-bit table are still the input's (nothing has
                 // written it since it was built): compress each slot to the open rows
                 int al[8];
@@ -580,7 +731,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                 }
                 if (lane < 2) ct[64 * W + lane] = 0u;
             } else {
-                // lane k < n_act takes open row k: its row index a_k through an LDS map (the
+                // lane k < n_act takes open row k: its row index through an LDS map (the
                 // 64-bit table is dead from here on), its words straight from the input
                 wave_sync();
                 if (me_act) dup[k_me] = (uint32_t)lane;
@@ -703,107 +854,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                     for (int j = 0; j < kFastW; j++)
                         if (j < W) dup[a_k * W + j] = x[j];
                 }
-                cnt = __shfl(cnt_k, k_me);
-            }
-        } else {
-            bool changed = (frozen & amask) != amask;
-            while (changed) {
-                pass++;
-                // per-pass copy of the group count that the compiler cannot hoist: the
-                // group guards are then re-tested with scalar compares instead of being
-                // kept (and spilled) as masks across the passes
-                int Gp = G;
-                asm volatile("" : "+s"(Gp));
-                uint32_t chg = 0;       // any implication written by this lane
-                // the reads of group g+1 are issued before group g applies its implications,
-                // so they may miss them until the next pass (the fixpoint is unchanged: the
-                // loop ends only after a pass without a single write, or with every row in
-                // conflict).  The empty asm keeps the unpacked literals inside the pass
-                // instead of hoisted into 48 live registers.
-                const auto fetch = [&](uint32_t (&s)[3], int g, u32x4& a, u32x4& b, u32x4& c) {
-                    uint32_t p = pk[g];
-                    asm volatile("" : "+v"(p));
-                    s[0] = dupb + 16u * (p & 1023u);
-                    s[1] = dupb + 16u * ((p >> 10) & 1023u);
-                    s[2] = dupb + 16u * (p >> 20);
-                    a = lds_read16(s[0]);
-                    b = lds_read16(s[1]);
-                    c = lds_read16(s[2]);
-                };
-                const auto group = [&](const uint32_t (&s)[3], const u32x4& qa, const u32x4& qb, const u32x4& qc) {
-                    const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
-                    const uint64_t T = u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y);
-                    const uint64_t open = ~(T | frozen);
-                    const uint64_t newconf = u64(and3(qa.z, qb.z, qc.z), and3(qa.w, qb.w, qc.w)) & open;
-                    const uint64_t U = u64(two3(qa.z, qb.z, qc.z), two3(qa.w, qb.w, qc.w)) & open;
-                    if (U) {
-                        const uint64_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
-                        if (Ua) {
-                            lds_or64(s[0], Ua);
-                            lds_or64((s[0] ^ 16u) + 8u, Ua);
-                        }
-                        if (Ub) {
-                            lds_or64(s[1], Ub);
-                            lds_or64((s[1] ^ 16u) + 8u, Ub);
-                        }
-                        if (Uc) {
-                            lds_or64(s[2], Uc);
-                            lds_or64((s[2] ^ 16u) + 8u, Uc);
-                        }
-                    }
-                    chg |= (uint32_t)U | (uint32_t)(U >> 32);
-                    if (ballot(newconf != 0)) {
-                        frozen |= wave_or64(newconf);
-                        return (frozen & amask) == amask;
-                    }
-                    return false;
-                };
-                uint32_t px[3], py[3];
-                u32x4 xa, xb, xc, ya, yb, yc;
-                fetch(px, 0, xa, xb, xc);
-#pragma unroll
-                for (int g = 0; g < kFastG; g += 2) {
-                    if (g < Gp) {
-                        if (g + 1 < kFastG) fetch(py, g + 1, ya, yb, yc);
-                        if (group(px, xa, xb, xc)) break;
-                    }
-                    if (g + 1 < Gp) {
-                        if (g + 2 < kFastG) fetch(px, g + 2, xa, xb, xc);
-                        if (group(py, ya, yb, yc)) break;
-                    }
-                }
-                changed = ballot(chg != 0) != 0ull && (frozen & amask) != amask;
-                wave_sync();
-            }
-            // complementary implications are conflicts too (nothing to look for once every
-            // row is in conflict)
-            uint64_t cmask = frozen;
-            if ((frozen & amask) != amask) {
-                uint64_t cm = 0;
-                for (int v = lane; v < 16 * W; v += kWave) {
-                    const u32x4 q = dup4[2 * v];
-                    cm |= u64(q.x, q.y) & u64(q.z, q.w);
-                }
-                cmask |= wave_or64(cm);
-            }
-            my_conflict = (cmask >> lane) & 1ull;
-            const bool all_conflict = (cmask & amask) == amask;
-            if (!all_conflict) {
-                uint32_t out[kFastW];
-#pragma unroll
-                for (int j = 0; j < kFastW; j++) {
-                    out[j] = 0u;
-                    if (j < W) {
-                        out[j] = tp(dup[4 * (32 * j + i) + half]);   // lane a: word j of row a
-                        cnt += __builtin_popcount(out[j]);
-                    }
-                }
-                wave_sync();
-                if (lane < A && !my_conflict) {
-#pragma unroll
-                    for (int j = 0; j < kFastW; j++)
-                        if (j < W) dup[lane * W + j] = out[j];
-                }
+                const int c_open = __shfl(cnt_k, k_me);
+                if (me_act) cnt = c_open;
             }
         }
         // the closure contains the input, so the implied literals are the difference of the
