@@ -872,9 +872,12 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
             // compiler does not keep the initial copy's addresses and guards alive (in
             // spilled scalar registers) through the whole kernel to reuse them.
             const uint32_t inv = (65536u + (uint32_t)W - 1u) / (uint32_t)W;
+            // (the pointer goes through the asm as an integer and comes back as a global-memory
+            // pointer, so the stores stay global stores rather than flat ones)
             int ln = lane, nw2 = nw;
-            uint32_t* dst2 = dst;
-            asm volatile("" : "+v"(ln), "+s"(nw2), "+s"(dst2));
+            uint64_t dst_bits = (uint64_t)(uintptr_t)dst;
+            asm volatile("" : "+v"(ln), "+s"(nw2), "+s"(dst_bits));
+            __attribute__((address_space(1))) uint32_t* dst2 = (__attribute__((address_space(1))) uint32_t*)dst_bits;
 #pragma unroll
             for (int u = 0; u < kFastW; u++) {
                 const int w = u * kWave + ln;
