@@ -82,7 +82,9 @@ int gpsat_tune_get(const char *key, int32_t *value);
  * Replaces the reference's per-formula device arena CUDAClauseVec::alloc_and_copy_to_dev
  * (src/Utils/CUDAClauseVec.cu:85-118) with one CSR arena for many instances:
  * lits[clause_off[c] .. clause_off[c+1]) is clause c, instance i owns clauses
- * inst_clause_off[i] .. inst_clause_off[i+1], n_vars[i] variables. */
+ * inst_clause_off[i] .. inst_clause_off[i+1], n_vars[i] variables.  Device memory: 4 B per
+ * literal, 4 B per clause offset and 32 B per instance; when every clause has 3 literals,
+ * n <= 256 and m <= 1024, also 4 B per clause of packed literal triples. */
 int gpsat_batch_create(const int32_t *lits, const int64_t *clause_off, const int64_t *inst_clause_off,
                        const int32_t *n_vars, int32_t n_inst, gpsat_batch **out);
 int gpsat_batch_destroy(gpsat_batch *b);
