@@ -287,15 +287,16 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
 // come from the batch's packed clause words (three 10-bit literal codes per 4-byte word).
 // Every global load of the instance (its A rows and its m clause words) is issued before
 // the first one is used; each lane then keeps its 16 clause words in VGPRs, so the BCP
-// passes touch LDS only.  With BCP, the input rows are copied
-// to the output as they arrive; only rows that gain implications are written again.
+// passes touch LDS only.  With BCP, the input rows are copied to the output as they
+// arrive; only rows that gain implications are written again.
 //
 // LDS holds the literal masks twice: slot l (16 bytes) = {T(l), T(l ^ 1)}, the true and
 // the false mask of literal l, so a clause literal costs one 16-byte read and no select.
 // A propagated literal l is OR-ed into T(l) of slot l and into the false half of slot
 // l ^ 1.  The first 64*W words double as the row staging area; slot 32*W is all zero.
-// With at most 32 open rows, BCP runs on a compacted table of 8-byte slots
-// {T(l), T(l ^ 1)} over the open rows, at words [64*W, 128*W + 2).
+// With at most 32 open rows (from the start, or once the 64-bit passes are down to 8-32
+// rows still gaining implications), BCP runs on a compacted table of 8-byte slots
+// {T(l), T(l ^ 1)} over those rows, at words [64*W, 128*W + 2).
 // LDS: (128*W + 8) words, so every wavefront's region starts 32-byte aligned (the BCP passes
 // find a slot's partner l ^ 1 by flipping one address bit).
 __device__ __forceinline__ int64_t uni64(int64_t x)
