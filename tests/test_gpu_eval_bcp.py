@@ -95,6 +95,32 @@ def test_fast3_limits(orc, A):
     assert (props[-1] == 199).all() and (conflict[-1] == 0).all()
 
 
+@pytest.mark.parametrize("A,k", [(64, 20), (40, 12), (64, 33), (48, 5)])
+def test_bcp_handoff(orc, A, k):
+    """A 200-link implication chain started in k of the A rows (x0 true there, the other rows
+    leave x0 open), beside a random instance at p = 0.1.  More than 32 rows are open, so the
+    passes start on the 64-bit table; the rows without the chain stop moving after the first
+    pass, and with 8 to 32 chain rows the passes continue on a compacted table of those rows
+    (k = 33: all 64-bit; k = 5: too few moving rows to hand off)."""
+    print(f"[eval/bcp] BCP handoff A={A} k={k}", flush=True)
+    from gpupsat_amd.formula import Formula
+    chain = [[2 * i, 2 * i + 3, 401] for i in range(198, -1, -1)]
+    fs = [Formula.from_clauses(201, chain), W.random_3sat_batch(1, 200, 852, seed=21).instance(0)]
+    b = Batch.from_formulas(fs)
+    t = np.zeros((A, 201), dtype=bool)
+    fl = np.zeros((A, 201), dtype=bool)
+    fl[:, 200] = True
+    t[:k, 0] = True
+    r0 = pack_rows(t, fl).reshape(-1)
+    rng = np.random.default_rng(k)
+    t1 = rng.random((A, 200)) < 0.05
+    f1 = (rng.random((A, 200)) < 0.05) & ~t1
+    rows = np.concatenate([r0, pack_rows(t1, f1).reshape(-1)])
+    with G.tuned(eval_fast3=1):
+        _, conflict, props = check_batch(orc, b, rows, A)
+    assert (props[0, :k] == 199).all() and (props[0, k:] == 0).all() and (conflict[0] == 0).all()
+
+
 @pytest.mark.parametrize("A", [1, 7, 33, 64])
 def test_mixed_sizes(orc, A):
     print(f"[eval/bcp] mixed k-SAT N=24 A={A}", flush=True)
