@@ -6,6 +6,8 @@
 
 #include <algorithm>
 #include <string>
+#include <thread>
+#include <vector>
 
 #include "../../include/gpsat.h"
 #include "gpsat_internal.hpp"
@@ -169,6 +171,49 @@ int gpsat_copy_dev(gpsat_batch* b, void* dst, const void* src, int64_t bytes)
     return GPSAT_OK;
 }
 
+// Packed clause words of one 3-literal instance (three 10-bit literal codes per word).  The
+// order of an instance's clauses, and of the literals inside a clause, changes no result of
+// the evaluation or BCP, so the words are laid out for the kernel's 16-byte LDS gathers: lane
+// c % 64 of a wavefront reads clause c, and for each 16-lane quarter and each of the three
+// positions a greedy pass prefers clauses (and literal orders) whose literal slots fall in
+// distinct residues mod 16, i.e. in distinct LDS bank groups (a 16-byte slot spans 4 of the
+// 64 banks).  Candidates come from the next 16 unplaced clauses.
+static void pack3_instance(const int32_t* L, int32_t m, uint32_t* out)
+{
+    static const int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
+    std::vector<char> used((size_t)std::max(m, 1), 0);
+    int32_t head = 0;
+    for (int32_t b = 0; b < m; b += 16) {
+        uint32_t ur[3] = {0, 0, 0};
+        for (int32_t ln = 0; ln < 16 && b + ln < m; ln++) {
+            while (head < m && used[head]) head++;
+            int32_t best = head;
+            int bp = 0, bc = 4, seen = 0;
+            for (int32_t c = head; c < m && seen < 16 && bc > 0; c++) {
+                if (used[c]) continue;
+                seen++;
+                for (int q = 0; q < 6 && bc > 0; q++) {
+                    int cost = 0;
+                    for (int k = 0; k < 3; k++) cost += (int)((ur[k] >> (L[3 * c + P[q][k]] & 15)) & 1u);
+                    if (cost < bc) {
+                        bc = cost;
+                        best = c;
+                        bp = q;
+                    }
+                }
+            }
+            used[best] = 1;
+            uint32_t w = 0;
+            for (int k = 0; k < 3; k++) {
+                const uint32_t l = (uint32_t)L[3 * best + P[bp][k]];
+                ur[k] |= 1u << (l & 15u);
+                w |= l << (10 * k);
+            }
+            out[b + ln] = w;
+        }
+    }
+}
+
 int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int64_t* inst_clause_off,
                        const int32_t* n_vars, int32_t n_inst, gpsat_batch** out)
 {
@@ -237,13 +282,22 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
     d.fast3 = fast3 && n_inst > 0;
     d.all3 = every3 && n_inst > 0;
     // the register-resident kernel reads one 4-byte word per clause instead of three
+    // (instances split over host threads)
     std::vector<uint32_t> h_pk3;
     if (d.fast3) {
         h_pk3.resize(M);
-        for (int64_t c = 0; c < M; c++) {
-            const int32_t* q = lits + clause_off[c0 + c];
-            h_pk3[c] = (uint32_t)q[0] | ((uint32_t)q[1] << 10) | ((uint32_t)q[2] << 20);
-        }
+        const int32_t T = (int32_t)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const int32_t nt = std::min(T, 1 + n_inst / 64);
+        const auto work = [&](int32_t t) {
+            for (int32_t i = t; i < n_inst; i += nt) {
+                const int64_t cb = inst_clause_off[i];
+                pack3_instance(lits + clause_off[cb], d.h_meta[i].m, h_pk3.data() + (cb - c0));
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int32_t t = 1; t < nt; t++) pool.emplace_back(work, t);
+        work(0);
+        for (std::thread& th : pool) th.join();
     }
     hipError_t e = hipGetDevice(&d.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
