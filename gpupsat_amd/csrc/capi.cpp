@@ -177,7 +177,7 @@ int gpsat_copy_dev(gpsat_batch* b, void* dst, const void* src, int64_t bytes)
 // c % 64 of a wavefront reads clause c, and for each 16-lane quarter and each of the three
 // positions a greedy pass prefers clauses (and literal orders) whose literal slots fall in
 // distinct residues mod 16, i.e. in distinct LDS bank groups (a 16-byte slot spans 4 of the
-// 64 banks).  Candidates come from the next 16 unplaced clauses.
+// 64 banks).  Candidates come from the next 48 unplaced clauses.
 static void pack3_instance(const int32_t* L, int32_t m, uint32_t* out)
 {
     static const int P[6][3] = {{0, 1, 2}, {0, 2, 1}, {1, 0, 2}, {1, 2, 0}, {2, 0, 1}, {2, 1, 0}};
@@ -189,7 +189,7 @@ static void pack3_instance(const int32_t* L, int32_t m, uint32_t* out)
             while (head < m && used[head]) head++;
             int32_t best = head;
             int bp = 0, bc = 4, seen = 0;
-            for (int32_t c = head; c < m && seen < 16 && bc > 0; c++) {
+            for (int32_t c = head; c < m && seen < 48 && bc > 0; c++) {
                 if (used[c]) continue;
                 seen++;
                 for (int q = 0; q < 6 && bc > 0; q++) {
