@@ -397,6 +397,10 @@ def main():
         parity = f"unchecked: {ex}"
 
     bytes_fused = N * algorithmic_bytes_per_instance(n, m, L, A, True, True)
+    # what the fused kernel itself must move with the batch's packed clause words (4 B per
+    # clause instead of the arena's 4L + 4(m+1)); the roofline keeps the arena-format figure
+    Wd = (2 * n + 31) // 32
+    bytes_fused_pk3 = N * (4 * m + 32 + A * 4 * Wd + A * 12 + A * (4 * Wd + 8))
     bytes_eval = N * algorithmic_bytes_per_instance(n, m, L, A, True, False)
     bytes_bcp = N * algorithmic_bytes_per_instance(n, m, L, A, False, True)
     achieved = bytes_fused / (ms_fused * 1e-3) / 1e9
@@ -448,6 +452,7 @@ def main():
             "eval_bcp_fused_ms": ms_fused, "eval_ms": ms_eval, "bcp_ms": ms_bcp,
             "eval_gbs": bytes_eval / (ms_eval * 1e-3) / 1e9, "bcp_gbs": bytes_bcp / (ms_bcp * 1e-3) / 1e9,
             "algorithmic_bytes_per_launch": {"fused": bytes_fused, "eval": bytes_eval, "bcp": bytes_bcp},
+            "packed_layout_bytes_per_launch": {"fused": bytes_fused_pk3},
             "copy_peak_gbs": copy_gbs,
         },
         "bcp": {
