@@ -27,35 +27,65 @@ HBM_PEAK_GBS = 8000.0
 METRIC = "verified instances/sec + BCP props/sec (HBM GB/s); learned split vs VSIDS"
 
 
-def algorithmic_bytes_per_instance(n: int, m: int, L: int, A: int, eval_: bool, bcp: bool) -> int:
-    """Compulsory HBM bytes of one instance: arena (4L literals + 4(m+1) offsets + 32 B
-    metadata), the input rows (A * 4W), and the outputs."""
-    W = (2 * n + 31) // 32
-    b = 4 * L + 4 * (m + 1) + 32 + A * 4 * W
+def survey_bytes_per_instance(n: int, m: int, L: int, A: int, eval_: bool, bcp: bool) -> int:
+    """Compulsory HBM bytes of one instance in the survey's accounting (the roofline's
+    denominator): the CSR arena 4(L + m + 1), the A input assignments at 2 bits per var
+    A*ceil(n/4), #SAT/#UNSAT/#UNIT out 12A, and the closure rows plus a conflict/props word
+    out A*(ceil(n/4) + 4).  n=200, m=852, L=2556, A=64: eval 17,604 B, eval+BCP 21,060 B."""
+    q = (n + 3) // 4
+    b = 4 * (L + m + 1) + A * q
     if eval_:
-        b += A * 12
+        b += 12 * A
     if bcp:
-        b += A * 4 * W + A * 8
+        b += A * (q + 4)
     return b
 
 
-def cpu_baseline_eval_bcp(batch, rows, A, seconds: float):
-    import numpy as np
+def packed_bytes_per_instance(n: int, m: int, A: int, eval_: bool, bcp: bool) -> int:
+    """What the 3-literal kernel itself must move: one 4-byte packed word per clause, 32 B of
+    metadata, the rows as stored (A * 4W in, A * 4W out) and the counts/flags."""
+    W = (2 * n + 31) // 32
+    b = 4 * m + 32 + A * 4 * W
+    if eval_:
+        b += 12 * A
+    if bcp:
+        b += A * (4 * W + 8)
+    return b
+
+
+def _eval_bcp_chunk(args):
+    """Worker: oracle clause counts + UP closure of a slice of instances (64 rows each)."""
+    batch, rows, A = args
+    import oracle
+    oracle.eval_batch(batch, rows, A)
+    oracle.bcp_batch(batch, rows, A)
+    return batch.n_inst
+
+
+def cpu_baseline_eval_bcp(batch, rows, A, seconds: float, cores: int):
+    """All-core CPU baseline of the headline step: P forked worker processes, each running
+    the oracle's clause evaluation and UP closure on its slices of the same instances and
+    rows.  The sample is sized from a one-core probe to take about `seconds` of wall time."""
+    import multiprocessing as mp
     import oracle
     base = batch.row_base(A)
-    done = 0
+
+    def piece(lo, hi):
+        idx = list(range(lo, hi))
+        return batch.subset(idx), np.ascontiguousarray(rows[base[lo]:base[hi]]), A
+
+    probe = piece(0, min(32, batch.n_inst))
     t0 = time.perf_counter()
-    i = 0
-    while time.perf_counter() - t0 < seconds and i < batch.n_inst:
-        idx = list(range(i, min(i + 4, batch.n_inst)))
-        sb = batch.subset(idx)
-        srows = np.concatenate([rows[base[j]:base[j + 1]] for j in idx])
-        oracle.eval_batch(sb, srows, A)
-        oracle.bcp_batch(sb, srows, A)
-        done += len(idx)
-        i += len(idx)
-    dt = time.perf_counter() - t0
-    return done / dt, done, dt
+    _eval_bcp_chunk(probe)
+    per = (time.perf_counter() - t0) / probe[0].n_inst
+    n_sample = int(min(batch.n_inst, max(cores * 8, seconds * cores / max(per, 1e-7))))
+    chunks = [piece(lo, min(n_sample, lo + 32)) for lo in range(0, n_sample, 32)]
+    ctx = mp.get_context("fork")
+    with ctx.Pool(cores) as pool:
+        t0 = time.perf_counter()
+        done = sum(pool.map(_eval_bcp_chunk, chunks, chunksize=1))
+        dt = time.perf_counter() - t0
+    return done / dt, done, dt, per
 
 
 def _ref_solve_chunk(sub):
@@ -111,13 +141,17 @@ def solve_leg(G, W, b, cpu):
     leg = {
         "workload": f"{N} uniform random 3-SAT (n=50, m=213), one wavefront per instance",
         "verified_instances_per_s": (verified_sat + int((st == 1).sum())) / (ms * 1e-3),
+        # the same count over the host's wall clock around gpsat_solve (job table set-up,
+        # copies, solve, model re-check and read-back included): the like-for-like figure
+        # beside the CPU pool's wall clock
+        "verified_instances_per_s_wall": (verified_sat + int((st == 1).sum())) / wall,
         "kernel_ms": inf["kernel_ms"], "verify_ms": inf["verify_ms"], "wall_s": wall,
         "sat": int((st == 0).sum()), "unsat": int((st == 1).sum()), "undef": int((st == 2).sum()),
         "sat_models_verified_on_gpu": verified_sat, "decided": decided,
         "verdict_agreement_sample": f"{agree}/{len(idx)}",
         "mean_conflicts": float(js[:, 1].mean()), "mean_props": float(js[:, 3].mean()),
         "props_per_s": float(js[:, 3].sum()) / (inf["kernel_ms"] * 1e-3),
-        "waves": inf["waves"], "clause_mem": {1: "LDS", 2: "HBM", 3: "split"}[inf["clause_mem"]],
+        "waves": inf["waves"], "clause_mem": {1: "LDS", 2: "HBM", 3: "split", 4: "HBM-all"}[inf["clause_mem"]],
     }
     # the reference's own sequential search on the GPU (trajectory mode), same instances
     gb.solve_ref(max_iterations=0, learned_cap=2048, learned_lits=16384)
@@ -149,6 +183,7 @@ def solve_leg(G, W, b, cpu):
                       f"{agree_cpu}/{sum(1 for r in cres if r != 2)} decided verdicts agree with the GPU",
         }
         leg["speedup_vs_cpu"] = leg["verified_instances_per_s"] / rate
+        leg["speedup_vs_cpu_wall"] = leg["verified_instances_per_s_wall"] / rate
     gb.close()
     return leg
 
@@ -202,7 +237,7 @@ def mixed_leg(G, W, N: int, deadline_ms: float, pool: int):
         "kernel_ms": ms, "epochs": inf["epochs"], "jobs": int(len(jobs[0])),
         "jobs_finished": int((js[:, 0] != 2).sum()),
         "props_per_s": float(js[:, 3].sum()) / (ms * 1e-3), "conflicts_per_s": float(js[:, 1].sum()) / (ms * 1e-3),
-        "launch_shape": {"waves": inf["waves"], "clause_mem": {1: "LDS", 2: "HBM", 3: "split"}[inf["clause_mem"]],
+        "launch_shape": {"waves": inf["waves"], "clause_mem": {1: "LDS", 2: "HBM", 3: "split", 4: "HBM-all"}[inf["clause_mem"]],
                          "chunk": inf["chunk"], "autotune_s": tune_s},
         "autotune_report": rep.strip().splitlines(),
     }
@@ -260,20 +295,64 @@ def learned_leg(n: int, cube_ms: float):
     return out
 
 
+def general_leg(G, W, N: int, steps: int):
+    """The general eval/BCP kernel (any clause length, n and m) on the mixed k-SAT batch of
+    the cube workload: 4,096 instances, k in {3,4,5} at ratios {4.26, 9.9, 21.1}, n in
+    [100, 400], 64 rows each.  GB/s by the survey's byte accounting per instance."""
+    b, ks = W.mixed_ksat_batch(N, seed=3)
+    A = 64
+    Ls = np.diff(b.clause_off[b.inst_clause_off])
+    ms_, ns_ = np.diff(b.inst_clause_off), b.n_vars
+    byts = int(sum(survey_bytes_per_instance(int(n), int(m), int(L), A, True, True)
+                   for n, m, L in zip(ns_, ms_, Ls)))
+    out = {"workload": f"{N} mixed random k-SAT (k 3/4/5 at 4.26/9.9/21.1, n 100-400, m up to {int(ms_.max())}), "
+                       f"{A} rows each, fused eval + BCP through the general kernel",
+           "survey_bytes_per_launch": byts, "densities": {}}
+    with G.GpuBatch(b) as gb:
+        for pd in (0.02, 0.05, 0.3):
+            rows = W.random_assignments(b, A, pd, seed=9)
+            d_r = G.DevBuf.from_host(rows)
+            d_c, d_o = G.DevBuf(N * A * 12), G.DevBuf(rows.nbytes)
+            d_f, d_p = G.DevBuf(N * A * 4), G.DevBuf(N * A * 4)
+            run = lambda: gb.eval_bcp_dev(d_r, A, d_c, d_o, d_f, d_p)
+            run()
+            gb.sync()
+            e0, e1 = G.Event(), G.Event()
+            e0.record(gb)
+            for _ in range(steps):
+                run()
+            e1.record(gb)
+            gb.sync()
+            ms = e0.elapsed_ms(e1) / steps
+            cf = d_f.to_host(np.int32, N * A)
+            pr = d_p.to_host(np.int32, N * A)
+            gbs = byts / (ms * 1e-3) / 1e9
+            out["densities"][f"p{pd}"] = {"fused_ms": ms, "instances_per_s": N / (ms * 1e-3), "survey_gbs": gbs,
+                                          "frac_of_8tbs": gbs / HBM_PEAK_GBS, "conflict_frac": float(cf.mean()),
+                                          "implied_literals_per_s": float(pr.sum()) / (ms * 1e-3)}
+            for d in (d_r, d_c, d_o, d_f, d_p):
+                d.close()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=20)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--instances", type=int, default=10_000)
-    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--p", type=float, default=0.3,
+                    help="trail density of the headline step (the workload's is 0.3; profiling runs at other "
+                         "densities set it with --no-sweeps so every eval/BCP launch is at that density)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--solve-instances", type=int, default=20_000)
     ap.add_argument("--no-solve", action="store_true", help="skip every solve leg")
     ap.add_argument("--no-legs", action="store_true", help="skip the mixed, structured and learned legs")
     ap.add_argument("--no-sweeps", action="store_true",
-                    help="skip the low-density BCP leg and the fused density sweep (every eval/BCP launch is then "
-                         "at p=0.3, so a profiler's per-kernel averages are the headline workload's)")
+                    help="skip the low-density BCP leg, the fused density sweep and the general-kernel leg (every "
+                         "3-literal eval/BCP launch is then at --p, so a profiler's per-kernel averages are the "
+                         "headline step's)")
     ap.add_argument("--mixed-instances", type=int, default=4096)
     ap.add_argument("--mixed-deadline-ms", type=float, default=20000.0)
     ap.add_argument("--learned-instances", type=int, default=256)
@@ -286,19 +365,27 @@ def main():
     import gpupsat_amd as G
     from gpupsat_amd import workloads as W
 
-    # The multi-process CPU baseline of the solve leg runs first, before this process
-    # touches the GPU, so the forked workers inherit no device state.
-    solve_batch = solve_cpu = None
-    if not args.no_solve:
-        solve_batch = W.random_3sat_batch(args.solve_instances, 50, 213, seed=100)
-        if not args.no_cpu:
-            cores = max(1, min(16, len(os.sched_getaffinity(0))))
-            solve_cpu = cpu_baseline_solve(solve_batch, args.cpu_seconds, cores) + (cores,)
-
-    N, n, m, A, p = args.instances, 200, 852, 64, 0.3
+    N, n, m, A, p = args.instances, 200, 852, 64, args.p
     b = W.random_3sat_batch(N, n, m, seed=0)
     rows = W.random_assignments(b, A, p, seed=0)
     L = int(b.clause_off[-1]) // N
+
+    # The multi-process CPU baselines run first, before this process touches the GPU, so
+    # the forked workers inherit no device state.
+    cores = max(1, min(16, len(os.sched_getaffinity(0))))
+    solve_batch = solve_cpu = cpu = None
+    if not args.no_cpu:
+        log(f"CPU baseline of the headline step on {cores} cores")
+        rate, done, dt, per = cpu_baseline_eval_bcp(b, rows, A, args.cpu_seconds, cores)
+        cpu = {"value": rate, "unit": "instances/s", "cores": cores, "kind": "port",
+               "sample": f"the oracle's clause evaluation + UP closure (oracle/oracle.c) of the first {done} of the "
+                         f"same instances x {A} rows, {dt:.1f} s wall on {cores} host processes "
+                         f"({1e3 * per:.2f} ms per instance on one core)"}
+    if not args.no_solve:
+        solve_batch = W.random_3sat_batch(args.solve_instances, 50, 213, seed=100)
+        if not args.no_cpu:
+            solve_cpu = cpu_baseline_solve(solve_batch, args.cpu_seconds, cores) + (cores,)
+
     gb = G.GpuBatch(b)
     d_rows = G.DevBuf.from_host(rows)
     d_counts = G.DevBuf(N * A * 12)
@@ -337,11 +424,12 @@ def main():
 
     ms_eval = timed(lambda: gb.clause_eval_dev(d_rows, A, d_counts), args.steps)
     ms_bcp = timed(lambda: gb.bcp_fixpoint_dev(d_rows, A, d_out, d_conf, d_props), args.steps)
-
+    step()
+    gb.sync()
     conflict = d_conf.to_host(np.int32, N * A).reshape(N, A)
     props = d_props.to_host(np.int32, N * A).reshape(N, A)
 
-    ms_bcp_lo, props_lo, conf_lo, sweep = None, None, None, None
+    ms_bcp_lo, props_lo, conf_lo, sweep, general = None, None, None, None, None
     if not args.no_sweeps:
         # low-density trails, where BCP actually implies literals (props/s)
         rows_lo = W.random_assignments(b, A, 0.05, seed=1)
@@ -361,11 +449,14 @@ def main():
             ms_p = timed(lambda: gb.eval_bcp_dev(d_p, A, d_counts, d_out, d_conf, d_props), args.steps)
             cf = d_conf.to_host(np.int32, N * A)
             pr = d_props.to_host(np.int32, N * A)
-            gbs = N * algorithmic_bytes_per_instance(n, m, L, A, True, True) / (ms_p * 1e-3) / 1e9
-            sweep[f"p{pd}"] = {"fused_ms": ms_p, "instances_per_s": N / (ms_p * 1e-3), "algorithmic_gbs": gbs,
-                               "frac_of_8tbs": gbs / 8000.0, "conflict_frac": float(cf.mean()),
-                               "implied_literals_per_s": float(pr.sum()) / (ms_p * 1e-3)}
+            gbs = N * survey_bytes_per_instance(n, m, L, A, True, True) / (ms_p * 1e-3) / 1e9
+            sweep[f"p{pd}"] = {"fused_ms": ms_p, "instances_per_s": N / (ms_p * 1e-3), "survey_gbs": gbs,
+                               "frac_of_8tbs": gbs / HBM_PEAK_GBS, "conflict_frac": float(cf.mean()),
+                               "implied_literals_per_s": float(pr.sum()) / (ms_p * 1e-3),
+                               "traffic_bytes": traffic_for(pd)}
             d_p.close()
+        log("general eval/BCP kernel on the mixed k-SAT batch")
+        general = general_leg(G, W, 4096, args.steps)
 
     # practical HBM peak: streaming copy of 2 GiB
     cb = 1 << 31
@@ -396,29 +487,12 @@ def main():
     except Exception as ex:  # the oracle is a checker; its absence must not hide the number
         parity = f"unchecked: {ex}"
 
-    bytes_fused = N * algorithmic_bytes_per_instance(n, m, L, A, True, True)
-    # what the fused kernel itself must move with the batch's packed clause words (4 B per
-    # clause instead of the arena's 4L + 4(m+1)); the roofline keeps the arena-format figure
-    Wd = (2 * n + 31) // 32
-    bytes_fused_pk3 = N * (4 * m + 32 + A * 4 * Wd + A * 12 + A * (4 * Wd + 8))
-    bytes_eval = N * algorithmic_bytes_per_instance(n, m, L, A, True, False)
-    bytes_bcp = N * algorithmic_bytes_per_instance(n, m, L, A, False, True)
+    bytes_fused = N * survey_bytes_per_instance(n, m, L, A, True, True)
+    bytes_eval = N * survey_bytes_per_instance(n, m, L, A, True, False)
+    bytes_bcp = N * survey_bytes_per_instance(n, m, L, A, False, True)
+    bytes_packed = N * packed_bytes_per_instance(n, m, A, True, True)
     achieved = bytes_fused / (ms_fused * 1e-3) / 1e9
-
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", "eval_bcp_3sat_n200_m852_A64_traffic.json")
-    if os.path.exists(tpath):
-        try:
-            traffic = json.load(open(tpath)).get("bytes_per_launch")
-        except Exception:
-            traffic = None
-
-    cpu = None
-    if not args.no_cpu:
-        rate, done, dt = cpu_baseline_eval_bcp(b, rows, A, args.cpu_seconds)
-        cpu = {"value": rate, "unit": "instances/s", "cores": 1, "kind": "port",
-               "sample": f"oracle clause-eval + UP-closure of the first {done} instances x 64 rows "
-                         f"(same workload), {dt:.1f} s on one host core"}
+    traffic = traffic_for(p)
 
     legs = {}
     if solve_batch is not None:
@@ -442,30 +516,53 @@ def main():
         "vs_baseline": None,
         "dtype": "int32",
         "data": "synthetic",
-        "config": {"workload": "batched clause-eval + BCP fixpoints: 10,000 random 3-SAT (n=200, m=852), "
-                               "64 partial assignments each (p=0.3)",
+        "config": {"workload": f"batched clause evaluation + BCP fixpoints: {N} random 3-SAT (n={n}, m={m}), "
+                               f"{A} partial assignments each (p={p}); value = instances whose {A} rows all got "
+                               f"exact #SAT/#UNSAT/#UNIT counts and exact BCP fixpoints, per second (no instance "
+                               f"is solved or verified in this step: the verified-instance rate of the solve path "
+                               f"is legs.solve)",
                    "instances": N, "n": n, "m": m, "assignments": A, "p": p},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic},
+                     "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "bytes_per_launch": bytes_fused,
+                     "bytes_rule": "survey accounting, 21,060 B per instance: arena 4(L+m+1), rows A*ceil(n/4) in, "
+                                   "counts 12A, closure rows and flags A*(ceil(n/4)+4) out",
+                     "kernel_compulsory_bytes": bytes_packed,
+                     "kernel_compulsory_gbs": bytes_packed / (ms_fused * 1e-3) / 1e9,
+                     "traffic_gbs": None if traffic is None else traffic / (ms_fused * 1e-3) / 1e9,
+                     "traffic_frac": None if traffic is None else traffic / (ms_fused * 1e-3) / 1e9 / HBM_PEAK_GBS},
         "cpu_baseline": cpu,
         "kernels": {
             "eval_bcp_fused_ms": ms_fused, "eval_ms": ms_eval, "bcp_ms": ms_bcp,
-            "eval_gbs": bytes_eval / (ms_eval * 1e-3) / 1e9, "bcp_gbs": bytes_bcp / (ms_bcp * 1e-3) / 1e9,
-            "algorithmic_bytes_per_launch": {"fused": bytes_fused, "eval": bytes_eval, "bcp": bytes_bcp},
-            "packed_layout_bytes_per_launch": {"fused": bytes_fused_pk3},
+            "eval_survey_gbs": bytes_eval / (ms_eval * 1e-3) / 1e9, "bcp_survey_gbs": bytes_bcp / (ms_bcp * 1e-3) / 1e9,
+            "survey_bytes_per_launch": {"fused": bytes_fused, "eval": bytes_eval, "bcp": bytes_bcp},
             "copy_peak_gbs": copy_gbs,
         },
         "bcp": {
-            "p0.3_conflict_frac": float(conflict.mean()),
+            f"p{p}_conflict_frac": float(conflict.mean()),
+            f"p{p}_implied_literals_per_s": float(props.sum()) / (ms_fused * 1e-3),
             "p0.05_ms": ms_bcp_lo,
             "p0.05_conflict_frac": None if conf_lo is None else float(conf_lo.mean()),
             "p0.05_props_per_s": None if props_lo is None else float(props_lo.sum()) / (ms_bcp_lo * 1e-3),
             "fused_density_sweep": sweep,
         },
+        "general_kernel": general,
         "parity_sample_ok": parity,
         "legs": legs,
     }
     print(json.dumps(out))
+
+
+def traffic_for(p: float):
+    """HBM bytes per fused launch at trail density p, from the committed rocprofv3 counter
+    summary (FETCH_SIZE and WRITE_SIZE, scaled by the copy-kernel calibration), or None."""
+    path = os.path.join(ROOT, "profiles", "eval_bcp_3sat_n200_m852_A64_traffic.json")
+    try:
+        d = json.load(open(path))
+    except Exception:
+        return None
+    by_p = d.get("bytes_per_launch_by_p") or {}
+    return by_p.get(f"p{p}")
 
 
 if __name__ == "__main__":

@@ -328,6 +328,7 @@ int gpsat_batch_destroy(gpsat_batch* b)
     (void)hipFree(d.lits);
     (void)hipFree(d.coff);
     (void)hipFree(d.pk3);
+    (void)hipFree(d.gtab);
     if (d.stream) (void)hipStreamDestroy(d.stream);
     delete b;
     return GPSAT_OK;

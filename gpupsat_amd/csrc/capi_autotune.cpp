@@ -1,11 +1,14 @@
 // capi_autotune.cpp -- the launch-shape autotuner (no reference counterpart: the reference
 // fixes its grid with -b/-t and its strategies at compile time, src/SATSolver/Configs.cuh).
 //
-// Solve path: short trial solves of a sample of the caller's jobs, one per candidate
-// shape (clause part in LDS or HBM, persistent wavefronts per CU, jobs per queue grab),
-// each under a wall-clock deadline; the measure is propagations per millisecond of
-// kernel time, i.e. the engine's work rate on this workload.  Trials never change
-// verdicts: every shape runs the same search per job.
+// Solve path: short trial solves, one per candidate shape (where the job state sits,
+// persistent wavefronts per CU, jobs per queue grab), each under a wall-clock deadline.  A
+// candidate is first resolved by solve_plan at the caller's full job count, exactly as the
+// full run would resolve it; the trial then runs that mode, wave count and chunk on a
+// sample of whole instances' job runs large enough that no wavefront idles, and only shapes
+// that were run can be returned.  The measure is finished jobs per millisecond of kernel
+// time (propagations per millisecond when too few jobs finish to compare).  Trials never
+// change verdicts: every shape runs the same search per job.
 //
 // Eval/BCP path: the caller's launch is timed under every shape (register-resident
 // 3-literal kernel or the general one, instances per workgroup, staged clause arena) and
@@ -21,6 +24,7 @@
 
 #include "../../include/gpsat.h"
 #include "gpsat_internal.hpp"
+#include "solver.hpp"
 
 using namespace gpsat;
 
@@ -47,56 +51,57 @@ struct Report {
     }
 };
 
-// The sampled job list: up to 256 instances spread evenly over the job order, and of each
-// a run of consecutive jobs (all of its jobs, or its first limit/instances cubes, so
-// chunked cube reuse is part of what is measured), limit jobs in all.
+// The sampled job list: whole runs of consecutive jobs of one instance (all the cubes the
+// caller queued for it, so chunked cube reuse is part of what is measured), every s-th run
+// of the caller's list so the sample spreads over the queue, about `target` jobs in all.
 struct Sample {
     std::vector<int32_t> inst, off, lits;
 };
 
 Sample sample_jobs(const int32_t* job_inst, const int32_t* cube_off, const int32_t* cube_lits, int32_t n_jobs,
-                   int32_t n_inst, int32_t limit)
+                   int32_t n_inst, int64_t target)
 {
     Sample s;
     s.off.push_back(0);
+    const int32_t total = job_inst ? n_jobs : n_inst;
+    const int64_t stride = std::max<int64_t>(1, (total + target - 1) / std::max<int64_t>(target, 1));
     if (!job_inst) {
-        // one job per instance, no cubes
-        const int32_t stride = std::max(1, n_inst / std::max(1, limit));
-        for (int32_t i = 0; i < n_inst && (int32_t)s.inst.size() < limit; i += stride) {
+        for (int32_t i = 0; i < n_inst; i += (int32_t)stride) {
             s.inst.push_back(i);
             s.off.push_back(0);
         }
         s.lits.push_back(0);
         return s;
     }
-    // runs of consecutive jobs of one instance
-    std::vector<std::pair<int32_t, int32_t>> runs;
-    for (int32_t j = 0; j < n_jobs;) {
+    int64_t r = 0;
+    for (int32_t j = 0; j < n_jobs; r++) {
         int32_t e = j + 1;
         while (e < n_jobs && job_inst[e] == job_inst[j]) e++;
-        runs.emplace_back(j, e);
+        if (r % stride == 0)
+            for (int32_t k = j; k < e; k++) {
+                s.inst.push_back(job_inst[k]);
+                for (int32_t q = cube_off[k]; q < cube_off[k + 1]; q++) s.lits.push_back(cube_lits[q]);
+                s.off.push_back((int32_t)s.lits.size());
+            }
         j = e;
-    }
-    const size_t want = std::min<size_t>(runs.size(), std::min<size_t>(256, (size_t)limit));
-    const size_t stride = std::max<size_t>(1, runs.size() / std::max<size_t>(1, want));
-    const int32_t per_run = std::max<int32_t>(1, limit / (int32_t)std::max<size_t>(1, want));
-    for (size_t r = 0; r < runs.size() && (int32_t)s.inst.size() < limit; r += stride) {
-        const int32_t e = std::min(runs[r].second, runs[r].first + per_run);
-        for (int32_t j = runs[r].first; j < e; j++) {
-            s.inst.push_back(job_inst[j]);
-            for (int32_t k = cube_off[j]; k < cube_off[j + 1]; k++) s.lits.push_back(cube_lits[k]);
-            s.off.push_back((int32_t)s.lits.size());
-        }
     }
     if (s.lits.empty()) s.lits.push_back(0);
     return s;
 }
 
+struct Trial {
+    SolvePlan plan;
+    double kernel_ms = 0;
+    int64_t props = 0, finished = 0;
+    int32_t jobs = 0;
+    bool ok = false;
+};
+
 }  // namespace
 
 extern "C" {
 
-static const char* mem_name(int mem) { return mem == 1 ? "LDS" : mem == 3 ? "split" : "HBM"; }
+static const char* mem_name(int mem) { return mem == 1 ? "LDS" : mem == 3 ? "split" : mem == 4 ? "HBM-all" : "HBM"; }
 
 int gpsat_autotune_solve(gpsat_batch* b, const gpsat_solve_opts* base, const int32_t* job_inst, const int32_t* cube_off,
                          const int32_t* cube_lits, int32_t n_jobs, int32_t sample_jobs_max, double trial_ms,
@@ -118,79 +123,151 @@ int gpsat_autotune_solve(gpsat_batch* b, const gpsat_solve_opts* base, const int
     }
     if (sample_jobs_max <= 0) sample_jobs_max = 4096;
     if (!(trial_ms > 0)) trial_ms = 300.0;
-    const Sample s = sample_jobs(job_inst, cube_off, cube_lits, n_jobs, N, sample_jobs_max);
-    const int32_t J = (int32_t)s.inst.size();
     const bool cubes = job_inst != nullptr;
+    const int32_t n_full = cubes ? n_jobs : N;
     int dev = 0, cus = 0;
     if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess) {
         set_error("no HIP device");
         return GPSAT_ERR_NODEVICE;
     }
-    std::vector<int32_t> status(N);
-    std::vector<int64_t> stats((size_t)J * GPSAT_JOB_STATS);
     Report rep;
-    rep.line("autotune solve: %d sampled jobs, trial deadline %.0f ms", J, trial_ms);
+    rep.line("autotune solve: %d jobs, trial deadline %.0f ms; every trial runs the shape the full run resolves to",
+             n_full, trial_ms);
 
-    // one trial: props per ms of kernel time (-1 if the shape does not fit)
-    auto trial = [&](int32_t mem, int32_t waves, int32_t chunk) -> double {
+    // The shape a full run of all n_full jobs launches with these choices.
+    auto plan_for = [&](int32_t mem, int32_t waves, int32_t chunk, SolvePlan& p) -> bool {
         gpsat_solve_opts t = o;
         t.clause_mem = mem;
         t.waves = waves;
         t.chunk = chunk;
-        t.verify = 0;
-        t.epoch_conflicts = 0;      // one launch per trial; epochs only replay the same kernel
-        t.deadline_ms = trial_ms;
+        SolveArgs a{};
+        return solve_plan(b->d, t, n_full, a, p) == GPSAT_OK;
+    };
+    // the candidate shapes of the first round: clause memory x wavefronts per CU
+    std::vector<SolvePlan> cand;
+    for (int32_t mem : {1, 2, 3})
+        for (int32_t per_cu : {8, 16, 24, 32}) {
+            SolvePlan p;
+            if (plan_for(mem, per_cu * cus, 0, p)) cand.push_back(p);
+        }
+    if (cand.empty()) {
+        set_error(std::string("no launch shape fits this batch: ") + gpsat_last_error());
+        rep.copy_to(report, report_len);
+        return GPSAT_ERR_CAPACITY;
+    }
+    // One sample for every trial, so their finished-job counts compare: at least two chunks
+    // per wavefront of the widest candidate (no wavefront idles before the deadline).
+    int64_t most = 0;
+    for (const SolvePlan& p : cand) most = std::max(most, (int64_t)p.waves * std::max(p.chunk, cubes ? 64 : 1));
+    const Sample s = sample_jobs(job_inst, cube_off, cube_lits, n_jobs, N, std::max<int64_t>(sample_jobs_max, 2 * most));
+    const int32_t J = (int32_t)s.inst.size();
+    rep.line("sample: %d jobs", J);
+    std::vector<Trial> done;
+    // One trial: the sample launched with the full run's mode, waves and chunk.
+    auto trial = [&](const SolvePlan& p) -> Trial {
+        Trial t;
+        t.plan = p;
+        for (const Trial& d : done)
+            if (d.plan.mode == p.mode && d.plan.waves == p.waves && d.plan.chunk == p.chunk) return d;
+        gpsat_solve_opts to = o;
+        to.clause_mem = p.mode;
+        to.waves = p.waves;
+        to.chunk = p.chunk;
+        to.verify = 0;
+        to.epoch_conflicts = 0;      // one launch per trial; epochs only replay the same kernel
+        to.deadline_ms = trial_ms;
+        to.learned_out = nullptr;
+        to.learned_cap = 0;
+        std::vector<int32_t> status(N);
+        std::vector<int64_t> stats((size_t)J * GPSAT_JOB_STATS);
         gpsat_solve_info inf;
-        const int rc = gpsat_solve(b, &t, s.inst.data(), s.off.data(), s.lits.data(), J, nullptr, nullptr, status.data(),
-                                   nullptr, nullptr, stats.data(), nullptr, &inf);
+        const int rc = gpsat_solve(b, &to, s.inst.data(), s.off.data(), s.lits.data(), J, nullptr, nullptr,
+                                   status.data(), nullptr, nullptr, stats.data(), nullptr, &inf);
         if (rc != GPSAT_OK) {
-            rep.line("  mem=%d waves=%d chunk=%d: does not fit (%s)", mem, waves, chunk, gpsat_last_error());
-            return -1.0;
+            rep.line("  mem=%s waves=%d chunk=%d: failed (%s)", mem_name(p.mode), p.waves, p.chunk, gpsat_last_error());
+            done.push_back(t);
+            return t;
         }
-        int64_t props = 0, done = 0;
+        if (inf.waves != p.waves || inf.chunk != p.chunk || inf.clause_mem != p.mode) {
+            rep.line("  mem=%s waves=%d chunk=%d: the trial ran waves=%d chunk=%d mem=%s instead; not ranked",
+                     mem_name(p.mode), p.waves, p.chunk, inf.waves, inf.chunk, mem_name(inf.clause_mem));
+            done.push_back(t);
+            return t;
+        }
         for (int32_t j = 0; j < J; j++) {
-            props += stats[(size_t)j * GPSAT_JOB_STATS + 3];
-            done += stats[(size_t)j * GPSAT_JOB_STATS] != 2;
+            t.props += stats[(size_t)j * GPSAT_JOB_STATS + 3];
+            const int64_t js = stats[(size_t)j * GPSAT_JOB_STATS];
+            t.finished += js == 0 || js == 1;
         }
-        const double rate = inf.kernel_ms > 0 ? (double)props / inf.kernel_ms : 0.0;
-        rep.line("  mem=%s waves=%d chunk=%d: %.0f props/ms, %lld/%d jobs finished, %.1f ms", mem_name(mem),
-                 inf.waves, inf.chunk, rate, (long long)done, J, inf.kernel_ms);
-        return rate;
+        t.kernel_ms = inf.kernel_ms;
+        t.jobs = J;
+        t.ok = inf.kernel_ms > 0;
+        rep.line("  mem=%s waves=%d chunk=%d: %lld finished (%.2f per ms), %.0f props/ms, %.1f ms", mem_name(p.mode),
+                 p.waves, p.chunk, (long long)t.finished, t.finished / std::max(t.kernel_ms, 1e-9),
+                 t.props / std::max(t.kernel_ms, 1e-9), t.kernel_ms);
+        done.push_back(t);
+        return t;
+    };
+    // Ranking: finished jobs (SAT or refuted) per ms of kernel time; when no trial of the
+    // round finished 32 jobs there are too few to compare and props per ms decides.
+    auto pick = [&](const std::vector<Trial>& ts) -> int {
+        int64_t most = 0;
+        for (const Trial& t : ts)
+            if (t.ok) most = std::max(most, t.finished);
+        const bool by_jobs = most >= 32;
+        int bi = -1;
+        double bs = -1;
+        for (int i = 0; i < (int)ts.size(); i++) {
+            if (!ts[i].ok) continue;
+            const double sc = (by_jobs ? (double)ts[i].finished : (double)ts[i].props) / ts[i].kernel_ms;
+            if (sc > bs) {
+                bs = sc;
+                bi = i;
+            }
+        }
+        return bi;
     };
 
-    // clause memory x wavefronts per CU (chunk automatic), then the chunk
-    double best_rate = -1.0;
-    int32_t bm = 2, bw = 0, bc = 0;
-    for (int32_t mem : {1, 2, 3}) {
-        for (int32_t per_cu : {8, 16, 24, 32}) {
-            const double r = trial(mem, per_cu * cus, 0);
-            if (r > best_rate) {
-                best_rate = r;
-                bm = mem;
-                bw = per_cu * cus;
-            }
-            if (r < 0) break;   // more wavefronts of this shape do not fit either
+    {   // warm-up launch (module load, first-touch of the queue buffers); not ranked
+        SolvePlan p;
+        if (plan_for(0, 0, 0, p)) {
+            gpsat_solve_opts to = o;
+            to.verify = 0;
+            to.epoch_conflicts = 0;
+            to.deadline_ms = std::min(trial_ms, 20.0);
+            to.learned_out = nullptr;
+            to.learned_cap = 0;
+            const Sample w = sample_jobs(job_inst, cube_off, cube_lits, n_jobs, N, 256);
+            std::vector<int32_t> status(N);
+            (void)gpsat_solve(b, &to, w.inst.data(), w.off.data(), w.lits.data(), (int32_t)w.inst.size(), nullptr,
+                              nullptr, status.data(), nullptr, nullptr, nullptr, nullptr, nullptr);
         }
     }
-    if (best_rate < 0) {
+    // clause memory x wavefronts per CU (automatic chunk), then the chunk
+    std::vector<Trial> round1;
+    for (const SolvePlan& p : cand) round1.push_back(trial(p));
+    int bi = pick(round1);
+    if (bi < 0) {
         set_error("no launch shape fits this batch");
         rep.copy_to(report, report_len);
         return GPSAT_ERR_CAPACITY;
     }
+    Trial win = round1[bi];
     if (cubes) {
+        std::vector<Trial> round2{win};
         for (int32_t chunk : {1, 8, 32, 64}) {
-            const double r = trial(bm, bw, chunk);
-            if (r > best_rate) {
-                best_rate = r;
-                bc = chunk;
-            }
+            SolvePlan p;
+            if (!plan_for(win.plan.mode, win.plan.waves, chunk, p)) continue;
+            round2.push_back(trial(p));
         }
+        const int b2 = pick(round2);
+        if (b2 >= 0) win = round2[b2];
     }
-    best->clause_mem = bm;
-    best->waves = bw;
-    best->chunk = bc;
-    rep.line("chosen: mem=%s waves=%d chunk=%s (%.0f props/ms)", mem_name(bm), bw,
-             bc ? std::to_string(bc).c_str() : "auto", best_rate);
+    best->clause_mem = win.plan.mode;
+    best->waves = win.plan.waves;
+    best->chunk = win.plan.chunk;
+    rep.line("chosen: mem=%s waves=%d chunk=%d (%lld finished jobs, %.0f props/ms in its trial)", mem_name(win.plan.mode),
+             win.plan.waves, win.plan.chunk, (long long)win.finished, win.props / std::max(win.kernel_ms, 1e-9));
     rep.copy_to(report, report_len);
     return GPSAT_OK;
 }

@@ -31,19 +31,30 @@ __device__ __forceinline__ void bulk_to_lds(uint32_t* dst, const uint32_t* __res
     for (; w < count; w += kWave) dst[w] = __builtin_nontemporal_load(src + w);
 }
 
-// Stage the instance's A rows (A*W words) in LDS and build LT (2 words per literal slot).
+// Orders this wavefront's accesses to its mask table: LDS (wave_sync), or, when the table
+// sits in an HBM slice (instances too large for LDS), after waiting for every memory
+// operation of the wavefront to complete.
+template <bool GTAB>
+__device__ __forceinline__ void table_sync()
+{
+    if (GTAB) __builtin_amdgcn_s_waitcnt(0);
+    wave_sync();
+}
+
+// Stage the instance's A rows (A*W words) and build LT (2 words per literal slot).
+template <bool GTAB>
 __device__ __forceinline__ void build_literal_masks(const uint32_t* __restrict__ rows, int A, int W,
                                                     uint32_t* stage, uint32_t* lt32, int lane, const Transposer& tp)
 {
     bulk_to_lds(stage, rows, A * W, lane);
-    wave_sync();
+    table_sync<GTAB>();
     const int half = lane >> 5, i = lane & 31;
     for (int j = 0; j < W; j++) {
         uint32_t x = (lane < A) ? stage[lane * W + j] : 0u;
         x = tp(x);
         lt32[2 * (32 * j + i) + half] = x;
     }
-    wave_sync();
+    table_sync<GTAB>();
 }
 
 struct ClauseMasks {
@@ -124,12 +135,16 @@ __device__ __forceinline__ void finish_counts(const uint32_t (&acc)[2][3], int l
 // DO_BCP:  write the closure rows, conflict flags and implied-literal counts.
 // STAGE: the instance's clause arena (literals and offsets) is copied to LDS first, so
 // every clause group reads LDS only.
-template <bool DO_EVAL, bool DO_BCP, bool STAGE>
+// GTAB: the mask table and row staging sit in the instance's slice of the HBM buffer
+// gtab (lds_words words per instance) instead of LDS: instances whose 2n literal masks do
+// not fit a workgroup's LDS.
+template <bool DO_EVAL, bool DO_BCP, bool STAGE, bool GTAB>
 __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ meta, const int32_t* __restrict__ lits,
                                                    const int32_t* __restrict__ coff, const uint32_t* __restrict__ rows,
                                                    int A, int32_t* __restrict__ counts, uint32_t* __restrict__ out_rows,
                                                    int32_t* __restrict__ conflict, int32_t* __restrict__ props,
-                                                   int32_t* __restrict__ passes_out, int n_inst, int lds_words)
+                                                   int32_t* __restrict__ passes_out, int n_inst, int lds_words,
+                                                   uint32_t* __restrict__ gtab)
 {
     extern __shared__ __align__(16) uint32_t lds_all[];
     // the wavefront's slot is uniform: its metadata then comes in with scalar loads
@@ -137,7 +152,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
     const int inst = blockIdx.x * (blockDim.x / kWave) + wslot;
     const int lane = threadIdx.x % kWave;
     if (inst >= n_inst) return;
-    uint32_t* smem = lds_all + (size_t)wslot * lds_words;
+    uint32_t* smem = GTAB ? gtab + (size_t)inst * lds_words : lds_all + (size_t)wslot * lds_words;
     const Transposer tp(lane);
     const InstMeta mi = meta[inst];
     const int W = __builtin_amdgcn_readfirstlane(words_per_row(mi.n));
@@ -160,8 +175,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
         icoff = s_coff;
     }
 
-    build_literal_masks(rows + row0, A, W, stage, lt32, lane, tp);
-    wave_sync();
+    build_literal_masks<GTAB>(rows + row0, A, W, stage, lt32, lane, tp);
+    table_sync<GTAB>();
 
     const int ngroups = (m + kWave - 1) / kWave;
     if (DO_EVAL) {
@@ -249,7 +264,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
                 }
             }
             changed = ballot(my_change) != 0ull && (frozen & amask) != amask;
-            wave_sync();
+            table_sync<GTAB>();
         }
         // complementary implications are conflicts too
         uint64_t cm = 0;
@@ -912,16 +927,19 @@ static int stage_lds_words(const DeviceBatch& b)
 template <bool E, bool B>
 static void launch_variant(const DeviceBatch& b, bool stage, int wpb, int lds_words, const uint32_t* rows, int A,
                            int32_t* counts, uint32_t* out_rows, int32_t* conflict, int32_t* props, int32_t* passes,
-                           hipStream_t st)
+                           uint32_t* gtab, hipStream_t st)
 {
     dim3 grid((b.n_inst + wpb - 1) / wpb), block(kWave * wpb);
     const size_t lds = (size_t)wpb * lds_words * 4;
-    if (stage)
-        hipLaunchKernelGGL((k_eval_bcp<E, B, true>), grid, block, lds, st, b.meta, b.lits, b.coff, rows, A, counts,
-                           out_rows, conflict, props, passes, b.n_inst, lds_words);
+    if (gtab)
+        hipLaunchKernelGGL((k_eval_bcp<E, B, false, true>), grid, block, 0, st, b.meta, b.lits, b.coff, rows, A, counts,
+                           out_rows, conflict, props, passes, b.n_inst, lds_words, gtab);
+    else if (stage)
+        hipLaunchKernelGGL((k_eval_bcp<E, B, true, false>), grid, block, lds, st, b.meta, b.lits, b.coff, rows, A,
+                           counts, out_rows, conflict, props, passes, b.n_inst, lds_words, gtab);
     else
-        hipLaunchKernelGGL((k_eval_bcp<E, B, false>), grid, block, lds, st, b.meta, b.lits, b.coff, rows, A, counts,
-                           out_rows, conflict, props, passes, b.n_inst, lds_words);
+        hipLaunchKernelGGL((k_eval_bcp<E, B, false, false>), grid, block, lds, st, b.meta, b.lits, b.coff, rows, A,
+                           counts, out_rows, conflict, props, passes, b.n_inst, lds_words, gtab);
 }
 
 // Wavefronts per workgroup (independent instances sharing a workgroup): the tuned value
@@ -935,9 +953,17 @@ static int waves_per_block(int lds_words)
     return w;
 }
 
+size_t eval_bcp_gtab_words(const DeviceBatch& b)
+{
+    if (b.n_inst == 0 || (b.fast3 && tuning().eval_fast3)) return 0;
+    const int lds_words = general_lds_words(b.max_n);
+    if ((size_t)lds_words * 4 <= 64 * 1024) return 0;
+    return (size_t)((lds_words + 63) & ~63) * b.n_inst;
+}
+
 hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, int32_t* counts, uint32_t* out_rows,
                            int32_t* conflict, int32_t* props, int32_t* passes, bool do_eval, bool do_bcp,
-                           hipStream_t st)
+                           hipStream_t st, uint32_t* gtab_ext)
 {
     if (b.n_inst == 0) return hipSuccess;
     if (b.fast3 && tuning().eval_fast3) {
@@ -959,14 +985,34 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
     // stage the arena in LDS when the largest instance keeps the wavefront under 40 KB
     const int staged = stage_lds_words(b);
     const bool stage = tuning().eval_stage_arena && (size_t)staged * 4 <= 40 * 1024;
-    const int lds_words = stage ? staged : general_lds_words(b.max_n);
-    const int wpb = waves_per_block(lds_words);
+    int lds_words = stage ? staged : general_lds_words(b.max_n);
+    // mask tables beyond a workgroup's 64 KB of LDS go to an HBM slice per instance, held
+    // by the batch from the first such launch on
+    uint32_t* gtab = nullptr;
+    if (!stage && (size_t)lds_words * 4 > 64 * 1024) {
+        lds_words = (lds_words + 63) & ~63;
+        const size_t need = (size_t)lds_words * b.n_inst;
+        if (gtab_ext) {
+            gtab = gtab_ext;
+        } else if (b.gtab_words < need) {
+            if (b.gtab) (void)hipFree(b.gtab);
+            b.gtab = nullptr;
+            b.gtab_words = 0;
+            hipError_t e = hipMalloc(&b.gtab, need * 4);
+            if (e != hipSuccess) return e;
+            b.gtab_words = need;
+        }
+        if (!gtab) gtab = b.gtab;
+    }
+    const int wpb = gtab ? 4 : waves_per_block(lds_words);
     if (do_eval && do_bcp)
-        launch_variant<true, true>(b, stage, wpb, lds_words, rows, A, counts, out_rows, conflict, props, passes, st);
+        launch_variant<true, true>(b, stage, wpb, lds_words, rows, A, counts, out_rows, conflict, props, passes, gtab, st);
     else if (do_eval)
-        launch_variant<true, false>(b, stage, wpb, lds_words, rows, A, counts, out_rows, conflict, props, passes, st);
+        launch_variant<true, false>(b, stage, wpb, lds_words, rows, A, counts, out_rows, conflict, props, passes, gtab,
+                                    st);
     else
-        launch_variant<false, true>(b, stage, wpb, lds_words, rows, A, counts, out_rows, conflict, props, passes, st);
+        launch_variant<false, true>(b, stage, wpb, lds_words, rows, A, counts, out_rows, conflict, props, passes, gtab,
+                                    st);
     return hipGetLastError();
 }
 

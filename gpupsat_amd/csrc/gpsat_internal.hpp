@@ -26,6 +26,10 @@ struct DeviceBatch {
     // device [n_clauses] when fast3: clause c's three literal codes in bits 0-9, 10-19 and
     // 20-29 (codes are below 2n <= 512); instance i's clauses start at meta.coff_base - i
     uint32_t* pk3 = nullptr;
+    // HBM slices for the eval/BCP mask tables of instances too large for LDS (allocated by
+    // the first launch that needs them, freed with the batch)
+    mutable uint32_t* gtab = nullptr;
+    mutable size_t gtab_words = 0;
     hipStream_t stream = nullptr;
     int device = 0;
     // host mirrors used by the solver set-up and the job chooser
@@ -51,9 +55,13 @@ constexpr int kFast3MaxGroups = 16;
 inline bool fast3_fits(int n, int m) { return words_per_row(n) <= kFast3MaxWords && m <= 64 * kFast3MaxGroups; }
 
 size_t eval_bcp_lds_bytes(int max_n);
+// gtab_ext: caller-provided HBM slices for the mask tables of instances too large for LDS
+// (eval_bcp_gtab_words of them); null: the batch's own, allocated on first use.
 hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, int32_t* counts, uint32_t* out_rows,
                            int32_t* conflict, int32_t* props, int32_t* passes, bool do_eval, bool do_bcp,
-                           hipStream_t st);
+                           hipStream_t st, uint32_t* gtab_ext = nullptr);
+// Words of HBM mask-table slices a general eval/BCP launch on this batch needs (0: none).
+size_t eval_bcp_gtab_words(const DeviceBatch& b);
 
 hipError_t launch_copy(const void* src, void* dst, int64_t bytes, hipStream_t st);
 

@@ -973,10 +973,11 @@ __device__ int32_t run_job(Ctx& J, const SolveArgs& a, const JobDesc& jd, int64_
 
 }  // namespace
 
-// kMode: where the clause part of a job's state sits -- 1 all in LDS, 2 all in this
-// wavefront's slice of the HBM workspace, 3 the watch lists and watch pairs of the original
-// clauses in LDS (read on every propagation step), the learned clauses' lists, pairs,
-// metadata and literals in HBM.
+// kMode: where a job's state sits -- 1 all in LDS, 2 the per-var state in LDS and the
+// clause part in this wavefront's slice of the HBM workspace, 3 the per-var state and the
+// watch lists and watch pairs of the original clauses in LDS (read on every propagation
+// step), the learned clauses' lists, pairs, metadata and literals in HBM, 4 everything in
+// the HBM slice (formulas whose per-var state does not fit a workgroup's LDS).
 template <int kMode>
 __global__ __launch_bounds__(kSolveWavesPerBlock * kWave) __attribute__((amdgpu_waves_per_eu(kSolveWavesPerEu, kSolveWavesPerEu)))
 void k_solve(SolveArgs a)
@@ -989,9 +990,15 @@ void k_solve(SolveArgs a)
     const int64_t gwave = (int64_t)blockIdx.x * (blockDim.x / kWave) + wslot;
     char* smem = smem_all + (size_t)wslot * a.lds_per_wave;
     Ctx J;
-    carve_state(J, smem, a.max_n);
-    int32_t* cpart = kMode == 1 ? (int32_t*)(smem + Layout::state_bytes(a.max_n))
-                                : a.ws + gwave * a.ws_ints_per_wave;
+    int32_t* cpart;
+    if (kMode == 4) {
+        char* slice = (char*)(a.ws + gwave * a.ws_ints_per_wave);
+        carve_state(J, slice, a.max_n);
+        cpart = (int32_t*)(slice + Layout::state_bytes(a.max_n));
+    } else {
+        carve_state(J, smem, a.max_n);
+        cpart = kMode == 1 ? (int32_t*)(smem + Layout::state_bytes(a.max_n)) : a.ws + gwave * a.ws_ints_per_wave;
+    }
     carve_clauses(J, cpart, a);
     if (kMode == 3) {
         int32_t* l = (int32_t*)(smem + Layout::state_bytes(a.max_n));
@@ -1044,7 +1051,15 @@ void k_solve(SolveArgs a)
                        ? (unsigned long long*)(a.job_stats + (int64_t)jd.order * JS_COUNT + JS_PH_SETUP) : nullptr;
         J.ph_t = t_job;
         J.ph_cur = PH_SETUP;
-        const int32_t st = (deadline && t_job > deadline) ? 2 : run_job(J, a, jd, budget, deadline);
+        // in epoch mode the epoch's budget replaces the per-job one, bounded by what is left
+        // of opts.max_conflicts after the job's earlier epochs (a spent job is UNDEF)
+        int64_t job_budget = budget;
+        if (a.epoch && a.o.max_conflicts > 0 && a.job_stats) {
+            const int64_t left = a.o.max_conflicts - a.job_stats[(int64_t)jd.order * JS_COUNT + JS_CONFLICTS];
+            job_budget = min(budget, left);
+        }
+        const bool spent = a.epoch && a.o.max_conflicts > 0 && a.job_stats && job_budget <= 0;
+        const int32_t st = ((deadline && t_job > deadline) || spent) ? 2 : run_job(J, a, jd, job_budget, deadline);
         phase(J, PH_SETUP);   // charge the last phase
         const uint64_t job_ticks = wall_clock64() - t_job;
         wave_mem_sync();
@@ -1126,8 +1141,9 @@ __global__ void k_epoch_end(SolveArgs a, int32_t* active, int32_t* n_active)
     for (int32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < n; k += gridDim.x * blockDim.x) {
         const int32_t job = cur[k];
         const JobDesc jd = a.jobs[job];
-        const int64_t st = a.job_stats[(int64_t)jd.order * JS_COUNT + JS_STATUS];
-        if (st == 2 && a.inst_status[jd.inst] == 2) next[atomicAdd(&n_active[(e + 1) & 1], 1)] = job;
+        const int64_t* js = a.job_stats + (int64_t)jd.order * JS_COUNT;
+        const bool spent = a.o.max_conflicts > 0 && js[JS_CONFLICTS] >= a.o.max_conflicts;
+        if (js[JS_STATUS] == 2 && !spent && a.inst_status[jd.inst] == 2) next[atomicAdd(&n_active[(e + 1) & 1], 1)] = job;
     }
 }
 
@@ -1166,6 +1182,8 @@ hipError_t launch_solve(const SolveArgs& a, int clause_mem, int waves, size_t ld
         hipLaunchKernelGGL(k_solve<1>, grid, block, lds_bytes * wpb, st, a);
     else if (clause_mem == 3)
         hipLaunchKernelGGL(k_solve<3>, grid, block, lds_bytes * wpb, st, a);
+    else if (clause_mem == 4)
+        hipLaunchKernelGGL(k_solve<4>, grid, block, 0, st, a);
     else
         hipLaunchKernelGGL(k_solve<2>, grid, block, lds_bytes * wpb, st, a);
     return hipGetLastError();

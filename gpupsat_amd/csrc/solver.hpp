@@ -118,9 +118,33 @@ int32_t solve_ref_wtot(int32_t n, int32_t L);
 hipError_t launch_solve_ref(const RefArgs& a, hipStream_t st);
 
 size_t solve_state_lds_bytes(int32_t max_n);
+
+// Launch plan of one solve (solve_plan in capi_solve.cpp): mode 1 all job state in LDS,
+// 2 clause part in HBM, 3 original clauses' watch structures in LDS and learned clauses in
+// HBM, 4 all in HBM; the persistent wavefronts, jobs per queue grab, and LDS per wavefront.
+struct SolvePlan {
+    int mode;
+    int waves;
+    int chunk;
+    int wpb;             // wavefronts per workgroup
+    int req_chunk;       // the caller's chunk (0: automatic)
+    int32_t n_jobs;
+    int cus;
+    int clock_khz;
+    size_t lds_bytes;
+};
 hipError_t launch_epoch_begin(const SolveArgs& a, hipStream_t st);
 hipError_t launch_epoch_end(const SolveArgs& a, int32_t* remaining, hipStream_t st);
 hipError_t launch_solve(const SolveArgs& a, int clause_mem, int waves, size_t lds_bytes, hipStream_t st);
 int solve_waves_per_block(size_t lds_bytes);
+
+}  // namespace gpsat
+
+struct gpsat_solve_opts;
+
+namespace gpsat {
+struct DeviceBatch;
+int solve_plan(const DeviceBatch& d, const gpsat_solve_opts& o, int32_t n_jobs, SolveArgs& a, SolvePlan& p);
+void set_waves(SolvePlan& p, int waves);
 
 }  // namespace gpsat

@@ -194,6 +194,7 @@ int orc_solve_cdcl(const int32_t *lits, const int64_t *clause_off, int32_t m, in
     for (int32_t v = 0; v < n; v++) h_insert(s, v);
     int status = ORC_UNDEF;
     int64_t conflicts = 0, decisions = 0, nlearnt = 0;
+    int32_t live = 0;   /* learned clauses not deleted */
     int32_t *lbuf = (int32_t *)malloc(4 * (2 * nv + 2));
     for (int32_t c = 0; c < m && status == ORC_UNDEF; c++) {
         int32_t len = (int32_t)(clause_off[c + 1] - clause_off[c]);
@@ -259,6 +260,7 @@ int orc_solve_cdcl(const int32_t *lits, const int64_t *clause_off, int32_t m, in
                 int32_t cid = add_clause(s, lbuf, nl, 1);
                 s->cact[cid] = s->cinc;
                 nlearnt++;
+                live++;
                 assign(s, lbuf[0], cid);
             }
             s->inc /= 0.95; s->cinc /= 0.999;
@@ -270,8 +272,6 @@ int orc_solve_cdcl(const int32_t *lits, const int64_t *clause_off, int32_t m, in
                 cancel_until(s, 0);
             }
             /* reduce learned DB */
-            int32_t live = 0;
-            for (int32_t c = 0; c < s->nc; c++) if (s->learnt[c] && !s->dead[c]) live++;
             if (live > max_learnt) {
                 int32_t *ids = (int32_t *)malloc(4 * live), k = 0;
                 for (int32_t c = 0; c < s->nc; c++) if (s->learnt[c] && !s->dead[c]) ids[k++] = c;
@@ -281,7 +281,7 @@ int orc_solve_cdcl(const int32_t *lits, const int64_t *clause_off, int32_t m, in
                     int32_t c = ids[i];
                     int32_t l0 = s->lits[s->cs[c]];
                     if (s->rsn[VAR(l0)] == c && val_lit(s, l0) == 1) continue;   /* locked */
-                    if (s->cn[c] > 2) s->dead[c] = 1;
+                    if (s->cn[c] > 2) { s->dead[c] = 1; live--; }
                 }
                 free(ids);
                 max_learnt += max_learnt / 10;

@@ -42,11 +42,17 @@ def test_autotune_solve_same_verdicts(orc):
     with G.GpuBatch(b) as gb:
         best, rep = gb.autotune_solve(base, jobs=jobs, trial_ms=150.0)
         assert "chosen:" in rep
-        assert best.waves > 0 and best.clause_mem in (1, 2, 3)
+        # the returned shape is explicit and is one a trial ran
+        assert best.waves > 0 and best.chunk > 0 and best.clause_mem in (1, 2, 3, 4)
+        ran = [ln for ln in rep.splitlines() if ln.startswith("  mem=") and "finished" in ln]
+        assert any(f"waves={best.waves} chunk={best.chunk}:" in ln for ln in ran), rep
         assert best.max_conflicts == 3000 and best.job_order == 1
         r1 = gb.solve(base, jobs=jobs)
         r2 = gb.solve(best, jobs=jobs)
         models = gb.models(r2["model_rows"])
+    # the full run launched the shape the autotuner measured
+    assert r2["info"]["waves"] == best.waves and r2["info"]["chunk"] == best.chunk
+    assert r2["info"]["clause_mem"] == best.clause_mem
     # budgets are per job and deterministic, so every decided verdict agrees
     for x, y in zip(r1["status"].tolist(), r2["status"].tolist()):
         assert x == y or G.UNDEF in (x, y)
@@ -54,5 +60,26 @@ def test_autotune_solve_same_verdicts(orc):
         if x == G.SAT:
             assert orc.verify_model(b.instance(i), models[i])
         elif x == G.UNSAT:
-            st, _, _ = orc.solve_cdcl(b.instance(i), max_conflicts=200_000)
-            assert st != G.SAT
+            # strict: the oracle's own CDCL must prove it (an oracle UNDEF fails the test)
+            st, _, _ = orc.solve_cdcl(b.instance(i), max_conflicts=2_000_000)
+            assert st == G.UNSAT, i
+
+
+def test_autotune_solve_no_cubes_runs_full_shape(orc):
+    """One job per instance: the wave count the tuner returns is the one the full run
+    uses, even when the sampled instances are fewer than the candidate wavefronts."""
+    print("[autotune] solve shapes, one job per instance", flush=True)
+    b = W.random_3sat_batch(3000, 50, 213, seed=8)
+    with G.GpuBatch(b) as gb:
+        best, rep = gb.autotune_solve(G.SolveOpts.default(), sample_jobs=512, trial_ms=60.0)
+        res = gb.solve(best, want_models=True)
+        check = gb.models(res["model_rows"])
+    assert res["info"]["waves"] == best.waves and res["info"]["chunk"] == best.chunk, rep
+    assert "not ranked" not in rep, rep
+    idx = range(0, b.n_inst, 150)
+    for i in idx:
+        st = res["status"][i]
+        exp, _, _ = orc.solve_cdcl(b.instance(i))
+        assert st == exp, i
+        if st == G.SAT:
+            assert orc.verify_model(b.instance(i), check[i])

@@ -81,3 +81,17 @@ def test_cli_trajectory_mode(orc, tmp_path):
              for l in lines]
         got = [list(map(int, x.groups())) for x in m if x][0]
         assert got == stats.tolist(), (i, got, stats)
+
+
+def test_cli_large_formula(orc, tmp_path):
+    """A 6,000-variable CNF (per-var solver state larger than the LDS) through the CLI:
+    SATISFIABLE with a verified model line, not a capacity error."""
+    print("[cli] n=6000 formula", flush=True)
+    f = W.random_3sat_batch(1, 6000, 18000, seed=61).instance(0)
+    p = tmp_path / "big.cnf"
+    p.write_text(f.to_dimacs())
+    lines = run("-i", str(p), "-b", "1", "-t", "1", "-v", "0")
+    assert verdict(lines) == "SATISFIABLE"
+    assert lines[-1] == "Solution was verified"
+    model = np.array([1 if int(x) > 0 else 0 for x in lines[-2].split()[:-1]], dtype=np.int8)
+    assert orc.verify_model(f, model)

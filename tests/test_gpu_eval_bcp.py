@@ -201,3 +201,19 @@ def test_3sat_n200_full_batch_properties(orc):
     np.testing.assert_array_equal(conflict[sample], e_conf)
     np.testing.assert_array_equal(props[sample], e_props)
     np.testing.assert_array_equal(np.concatenate([out[base[i]:base[i + 1]] for i in sample]), e_out)
+
+
+@pytest.mark.parametrize("A", [1, 8, 64])
+def test_large_instances_hbm_mask_table(orc, A):
+    """Instances whose 2n literal masks exceed a workgroup's LDS (n = 2,500 and 5,000,
+    beside a small one): the general kernel keeps its mask table in an HBM slice per
+    instance, with the same bit-exact counts and fixpoints."""
+    print(f"[eval/bcp] n=2500/5000 mask tables in HBM, A={A}", flush=True)
+    fs = [W.random_3sat_batch(1, 2500, 7500, seed=31).instance(0),
+          W.random_3sat_batch(1, 60, 255, seed=32).instance(0),
+          W.mixed_ksat_batch(1, seed=33)[0].instance(0),
+          W.random_3sat_batch(1, 5000, 21000, seed=34).instance(0)]
+    b = Batch.from_formulas(fs)
+    rows = W.random_assignments(b, A, 0.02, seed=35)
+    counts, conflict, props = check_batch(orc, b, rows, A)
+    assert props.sum() > 0

@@ -294,3 +294,55 @@ def test_learned_clauses_are_implied(orc, mem):
     assert checked > 100 and not_implied > 0
     print(f"[solve] {checked} learned clauses checked ({units} level-0 units); "
           f"{not_implied} random clauses correctly found not implied", flush=True)
+
+
+def test_epoch_budget_respects_max_conflicts(orc):
+    """With restart epochs, a job's conflicts over all its epochs stay within the per-job
+    max_conflicts; jobs that reach it end UNDEF and are not re-queued."""
+    print("[solve] epoch budget bounded by max_conflicts", flush=True)
+    fs = [W.pigeonhole_formula(8, 7)] * 4
+    fs += [W.random_3sat_batch(1, 60, 256, seed=900 + i).instance(0) for i in range(20)]
+    b = Batch.from_formulas(fs)
+    cap = 700
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(max_ms=0.0, max_conflicts=cap, epoch_conflicts=100, epoch_growth=2.0, max_epochs=12))
+        check_models(orc, gb, res, b)
+    js = res["job_stats"]
+    assert (js[:, 1] <= cap).all(), js[:, 1].max()
+    assert (res["status"][:4] == G.UNDEF).all()          # PHP(8,7) needs far more than 700 conflicts
+    assert (js[:4, 1] == cap).all()
+    for i in range(4, b.n_inst):
+        st = res["status"][i]
+        if st != G.UNDEF:
+            assert st == truth(orc, b.instance(i))
+        else:
+            assert js[i, 1] == cap
+
+
+def test_all_hbm_state_mode(orc):
+    """clause_mem=4 keeps the per-var state in HBM too: verdicts equal the oracle's, the
+    same as the all-LDS layout gives."""
+    print("[solve] all-HBM job state, n=50", flush=True)
+    b = W.random_3sat_batch(60, 50, 213, seed=41)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(clause_mem=4))
+        check_models(orc, gb, res, b)
+        ref = gb.solve(opts(clause_mem=1))
+    assert res["info"]["clause_mem"] == 4 and res["info"]["lds_bytes"] == 0
+    assert res["status"].tolist() == [truth(orc, b.instance(i)) for i in range(b.n_inst)]
+    assert res["status"].tolist() == ref["status"].tolist()
+
+
+def test_large_formula_beyond_lds(orc):
+    """Formulas whose per-var state exceeds a workgroup's LDS (n = 4,000 and 12,000) are
+    solved with the whole job state in HBM instead of being refused."""
+    print("[solve] n=4000/12000 with the job state in HBM", flush=True)
+    fs = [W.random_3sat_batch(1, 4000, 12000, seed=51).instance(0),
+          W.random_3sat_batch(1, 12000, 36000, seed=52).instance(0),
+          W.pigeonhole_formula(6, 5)]
+    b = Batch.from_formulas(fs)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(max_ms=20000.0))
+        check_models(orc, gb, res, b)
+    assert res["info"]["clause_mem"] == 4
+    assert res["status"].tolist() == [G.SAT, G.SAT, G.UNSAT]
