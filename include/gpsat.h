@@ -241,6 +241,23 @@ int gpsat_solve(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t *job_in
                 int32_t *status, uint32_t *model_rows, int32_t *verified, int64_t *job_stats, int32_t *var_bumps,
                 gpsat_solve_info *info);
 
+/* Allocation-free solve (no counterpart in the reference): gpsat_solve for callers that
+ * cannot allocate device memory -- code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc.  Same arguments, outputs and verdicts as gpsat_solve; every
+ * device buffer (job table, queue and instance counters, job statistics, model rows, the
+ * solver workspace, the model check's counts) is carved from one static device region of
+ * GPSAT_NOALLOC_SCRATCH_MIB MiB per GPU, which the library holds until the process ends; no
+ * hipMalloc, hipFree or pinned allocation is made.  When the clause part of the job state
+ * sits in HBM, fewer persistent wavefronts are launched if their slices would not fit.  A call
+ * whose buffers exceed the region returns GPSAT_ERR_CAPACITY before anything runs; restart
+ * epochs (epoch_conflicts > 0) and the learned-clause dump return GPSAT_ERR_ARG.  Every call
+ * on a GPU shares that GPU's region: calls that overlap on one GPU race. */
+#define GPSAT_NOALLOC_SCRATCH_MIB 64
+int gpsat_solve_noalloc(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t *job_inst, const int32_t *cube_off,
+                        const int32_t *cube_lits, int32_t n_jobs, const float *act_init, const int8_t *phase_init,
+                        int32_t *status, uint32_t *model_rows, int32_t *verified, int64_t *job_stats,
+                        int32_t *var_bumps, gpsat_solve_info *info);
+
 /* Trajectory mode: the reference's own sequential search per instance, step for step
  * (SATSolver::solve, src/SATSolver/SATSolver.cu:67-218 with the default strategies of
  * Configs.cuh: last-free-var decisions, decision-scheme learning, 20-clause round-robin
