@@ -292,6 +292,12 @@ def learned_leg(n: int, cube_ms: float):
     out = T.compare(b, theta, opts, cube_depth=4)
     out["workload"] = f"{n} held-out random 3-SAT (n=300, m=1278), 16 cubes each"
     out["weights"] = "gpupsat_amd/branching_weights.npy (trained on the GPU by gpsat_gnn_train_step)"
+    from gpupsat_amd import learned as Lr
+    ms, flops = Lr.time_train_steps(n_inst=512, steps=10)
+    out["train_step"] = {"what": "one training step (forward, loss, backward, Adam) on 512 random 3-SAT instances "
+                                 "(n=300, m=1278), wall clock around the synchronous call",
+                         "ms": ms, "algorithmic_tflops": flops / 1e12, "frac_of_bf16_peak": flops / 2.5e15,
+                         "flops_rule": "3 x 512 x 3 rounds x (2n+m) nodes x 2*64*64*2"}
     return out
 
 

@@ -15,7 +15,9 @@ import numpy as np
 from .formula import Batch, Formula, pack_rows, unpack_rows, words_per_row  # noqa: F401
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libgpsat.so")
+# GPSAT_LIB selects another build of the library (the sanitizer build of
+# tests/run_sanitized.sh)
+LIB_PATH = os.environ.get("GPSAT_LIB") or os.path.join(_HERE, "libgpsat.so")
 CLI_PATH = os.path.join(_HERE, "bin", "gpupsat")
 HEADER_PATH = os.path.join(os.path.dirname(_HERE), "include", "gpsat.h")
 

@@ -37,9 +37,17 @@ COMMON = [
 ]
 
 
-def _compile(src: str) -> str:
-    obj = os.path.join(OBJ, os.path.basename(src) + ".o")
-    cmd = [_hipcc(), *COMMON, "-c", src, "-o", obj]
+# Host-side AddressSanitizer + UndefinedBehaviorSanitizer (device code is built as usual):
+# each -fsanitize= goes to the host pass only.
+SAN_HOST = ["-Xarch_host", "-fsanitize=address", "-Xarch_host", "-fsanitize=undefined",
+            "-Xarch_host", "-fno-omit-frame-pointer", "-Xarch_host", "-g"]
+SAN_OBJ = os.path.join(HERE, "_build_san")
+SAN_LIB = os.path.join(SAN_OBJ, "libgpsat.so")
+
+
+def _compile(src: str, sanitize: bool = False) -> str:
+    obj = os.path.join(SAN_OBJ if sanitize else OBJ, os.path.basename(src) + ".o")
+    cmd = [_hipcc(), *COMMON, *(SAN_HOST if sanitize else []), "-c", src, "-o", obj]
     if src.endswith(".cpp"):
         cmd.insert(1, "-x")
         cmd.insert(2, "hip")
@@ -80,5 +88,23 @@ def build(verbose: bool = False, jobs: int | None = None) -> str:
     return LIB
 
 
+def build_sanitized(jobs: int | None = None) -> str:
+    """libgpsat.so with sanitized host code, in gpupsat_amd/_build_san/ (CPU testing only)."""
+    os.makedirs(SAN_OBJ, exist_ok=True)
+    srcs = sources()
+    jobs = jobs or min(len(srcs), int(os.environ.get("MAX_JOBS", os.cpu_count() or 4)))
+    with cf.ThreadPoolExecutor(max_workers=max(1, jobs)) as ex:
+        objs = list(ex.map(lambda s: _compile(s, True), srcs))
+    cmd = [_hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", "-shared-libasan",
+           "-Xarch_host", "-fsanitize=address,undefined", "-o", SAN_LIB, *objs]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"link failed: {' '.join(cmd)}\n{r.stdout}\n{r.stderr}")
+    return SAN_LIB
+
+
 if __name__ == "__main__":
-    build(verbose=True)
+    if "--sanitize" in sys.argv:
+        print(build_sanitized())
+    else:
+        build(verbose=True)

@@ -8,13 +8,16 @@
 //   M_l = 1/8 sum_{c ni l} H_c'    H_l' = relu([H_l | M_l | H_{not l}] W_l^T + b_l)
 //   Z_v = [H_{2v+1} | H_{2v}]      R_v = relu(Z_v W_o^T + b_o)   Y_v = R_v W_y^T + b_y
 // Y_v = (branching score, phase logit).  The three 64-wide products run on MFMA
-// (v_mfma_f32_16x16x16_bf16): BF16 weights and BF16 inputs, FP32 accumulation.
-// Aggregations, biases, the 2-output head, the loss and Adam are FP32.
+// (v_mfma_f32_16x16x32_bf16): BF16 weights and BF16 inputs, FP32 accumulation.
+// Aggregations, biases, the 2-output head, the loss and Adam are FP32.  Gradients and the
+// loss are summed from per-wavefront (or per-workgroup) partials in a fixed order, never
+// with floating-point atomics, so they are reproducible bit for bit.
 #include <hip/hip_runtime.h>
 #include <string.h>
 
 #include <algorithm>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "../../include/gpsat.h"
@@ -44,153 +47,187 @@ __device__ __forceinline__ bf16_t to_bf16(float f)
 __device__ __forceinline__ float from_bf16(bf16_t h) { return __uint_as_float((uint32_t)h << 16); }
 
 // ---------------------------------------------------------------------------------------
-// MFMA tiles.  v_mfma_f32_16x16x16_bf16 on a wave of 64: lane l supplies
-// A[l%16][4(l/16)+j] and B[4(l/16)+j][l%16] (j = 0..3) and receives D[4(l/16)+j][l%16].
+// MFMA tiles.  v_mfma_f32_16x16x32_bf16 (gfx950) on a wave of 64: lane l supplies the 8
+// consecutive-k values A[l%16][8(l/16) .. 8(l/16)+7] and B[8(l/16) .. 8(l/16)+7][l%16] as one
+// 16-byte read each, and receives D[4(l/16)+j][l%16] (j = 0..3).  LDS tiles are stored
+// with the k index contiguous (transposed where the source is not), rows padded by 16 bytes
+// so the 16 rows a wave reads at once fall in different banks.
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ bf16x8 ld8(const bf16_t* p) { return __builtin_bit_cast(bf16x8, *reinterpret_cast<const uint4*>(p)); }
+__device__ __forceinline__ f32x4 mfma32(bf16x8 a, bf16x8 b, f32x4 c)
+{
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b, c, 0, 0, 0);
+}
 
 // out[N x 64] = relu(X[N x K] W[64 x K]^T + b); one wave per 16 rows, W staged in LDS.
 template <int K>
 __global__ __launch_bounds__(256) void k_dense_fwd(const bf16_t* __restrict__ X, const bf16_t* __restrict__ Wq,
                                                    const float* __restrict__ bias, float* __restrict__ out, int N)
 {
-    __shared__ __align__(16) bf16_t sW[64 * K];
-    for (int i = threadIdx.x; i < 64 * K / 4; i += blockDim.x)
-        reinterpret_cast<uint2*>(sW)[i] = reinterpret_cast<const uint2*>(Wq)[i];
+    constexpr int SK = K + 8;
+    __shared__ __align__(16) bf16_t sW[64 * SK];
+    for (int i = threadIdx.x; i < 64 * K / 8; i += blockDim.x) {
+        const int r = (8 * i) / K, c = (8 * i) % K;
+        *reinterpret_cast<uint4*>(sW + r * SK + c) = reinterpret_cast<const uint4*>(Wq)[i];
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r0 = (blockIdx.x * 4 + wave) * 16;
     if (r0 >= N) return;
-    const int ar = r0 + (lane & 15), kq = 4 * (lane >> 4);
+    const int ar = r0 + (lane & 15), kq = 8 * (lane >> 4);
     f32x4 acc[4];
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int k0 = 0; k0 < K; k0 += 16) {
-        bf16x4 a = {0, 0, 0, 0};
-        if (ar < N) a = *reinterpret_cast<const bf16x4*>(X + (size_t)ar * K + k0 + kq);
+    for (int k0 = 0; k0 < K; k0 += 32) {
+        bf16x8 a = __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
+        if (ar < N) a = ld8(X + (size_t)ar * K + k0 + kq);
 #pragma unroll
-        for (int ct = 0; ct < 4; ct++) {
-            const bf16x4 b = *reinterpret_cast<const bf16x4*>(sW + (ct * 16 + (lane & 15)) * K + k0 + kq);
-            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct], 0, 0, 0);
-        }
+        for (int ct = 0; ct < 4; ct++) acc[ct] = mfma32(a, ld8(sW + (ct * 16 + (lane & 15)) * SK + k0 + kq), acc[ct]);
     }
+    const int dq = 4 * (lane >> 4);
 #pragma unroll
     for (int ct = 0; ct < 4; ct++) {
         const int col = ct * 16 + (lane & 15);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int row = r0 + kq + j;
+            const int row = r0 + dq + j;
             if (row < N) out[(size_t)row * 64 + col] = fmaxf(acc[ct][j] + bias[col], 0.0f);
         }
     }
 }
 
-// dX[N x K] = dP[N x 64] W[64 x K]
+// dX[N x K] = dP[N x 64] W[64 x K]; W staged transposed (k = hidden index contiguous).
 template <int K>
 __global__ __launch_bounds__(256) void k_dense_bwd_dx(const bf16_t* __restrict__ dP, const bf16_t* __restrict__ Wq,
                                                       float* __restrict__ dX, int N)
 {
-    __shared__ __align__(16) bf16_t sW[64 * K];
-    for (int i = threadIdx.x; i < 64 * K / 4; i += blockDim.x)
-        reinterpret_cast<uint2*>(sW)[i] = reinterpret_cast<const uint2*>(Wq)[i];
+    constexpr int SH = 64 + 8;
+    __shared__ __align__(16) bf16_t sWt[K * SH];
+    for (int i = threadIdx.x; i < 64 * K; i += blockDim.x) {
+        const int h = i / K, c = i % K;
+        sWt[c * SH + h] = Wq[i];
+    }
     __syncthreads();
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int r0 = (blockIdx.x * 4 + wave) * 16;
     if (r0 >= N) return;
-    const int ar = r0 + (lane & 15), kq = 4 * (lane >> 4);
+    const int ar = r0 + (lane & 15), kq = 8 * (lane >> 4);
     constexpr int CT = K / 16;
     f32x4 acc[CT];
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-    for (int h0 = 0; h0 < 64; h0 += 16) {
-        bf16x4 a = {0, 0, 0, 0};
-        if (ar < N) a = *reinterpret_cast<const bf16x4*>(dP + (size_t)ar * 64 + h0 + kq);
+    for (int h0 = 0; h0 < 64; h0 += 32) {
+        bf16x8 a = __builtin_bit_cast(bf16x8, uint4{0u, 0u, 0u, 0u});
+        if (ar < N) a = ld8(dP + (size_t)ar * 64 + h0 + kq);
 #pragma unroll
-        for (int ct = 0; ct < CT; ct++) {
-            const int col = ct * 16 + (lane & 15);
-            bf16x4 b;
-            b[0] = (short)sW[(h0 + kq + 0) * K + col];
-            b[1] = (short)sW[(h0 + kq + 1) * K + col];
-            b[2] = (short)sW[(h0 + kq + 2) * K + col];
-            b[3] = (short)sW[(h0 + kq + 3) * K + col];
-            acc[ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[ct], 0, 0, 0);
-        }
+        for (int ct = 0; ct < CT; ct++) acc[ct] = mfma32(a, ld8(sWt + (ct * 16 + (lane & 15)) * SH + h0 + kq), acc[ct]);
     }
+    const int dq = 4 * (lane >> 4);
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) {
         const int col = ct * 16 + (lane & 15);
 #pragma unroll
         for (int j = 0; j < 4; j++) {
-            const int row = r0 + kq + j;
+            const int row = r0 + dq + j;
             if (row < N) dX[(size_t)row * K + col] = acc[ct][j];
         }
     }
 }
 
-// gW[64 x K] += dP[N x 64]^T X[N x K]; one wave per block, 64-node chunks staged in LDS.
+// Partial weight gradient of this workgroup: part[blockIdx.x][64 x K] = sum over its
+// 64-node chunks of dP[N x 64]^T X[N x K] (the reduction adds the workgroups' partials in
+// order).  A chunk is staged transposed in LDS (node index contiguous: sPt[h][n], sXt[col][n])
+// by all four wavefronts; wavefront w then computes output rows h = 16w .. 16w+15.
 template <int K>
-__global__ __launch_bounds__(64) void k_dense_bwd_dw(const bf16_t* __restrict__ dP, const bf16_t* __restrict__ X,
-                                                     float* __restrict__ gW, int N)
+__global__ __launch_bounds__(256) void k_dense_bwd_dw(const bf16_t* __restrict__ dP, const bf16_t* __restrict__ X,
+                                                      float* __restrict__ part, int N)
 {
-    __shared__ __align__(16) bf16_t sP[64 * 64];
-    __shared__ __align__(16) bf16_t sX[64 * K];
+    constexpr int SN = 64 + 8;
+    __shared__ __align__(16) bf16_t sPt[64 * SN];
+    __shared__ __align__(16) bf16_t sXt[K * SN];
     constexpr int CT = K / 16;
-    const int lane = threadIdx.x;
-    const int kq = 4 * (lane >> 4);
-    f32x4 acc[4][CT];
+    const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
+    const int kq = 8 * (lane >> 4);
+    f32x4 acc[CT];
 #pragma unroll
-    for (int ht = 0; ht < 4; ht++)
-#pragma unroll
-        for (int ct = 0; ct < CT; ct++) acc[ht][ct] = f32x4{0.f, 0.f, 0.f, 0.f};
+    for (int ct = 0; ct < CT; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int n0 = blockIdx.x * 64; n0 < N; n0 += gridDim.x * 64) {
         const int rows = min(64, N - n0);
         __syncthreads();
-        for (int i = lane; i < 64 * 64 / 4; i += 64) {
-            const int r = (i * 4) / 64;
+        for (int i = t; i < 64 * 64 / 4; i += 256) {
+            const int r = (4 * i) / 64, h = (4 * i) % 64;
             uint2 v = {0u, 0u};
             if (r < rows) v = reinterpret_cast<const uint2*>(dP + (size_t)n0 * 64)[i];
-            reinterpret_cast<uint2*>(sP)[i] = v;
+            sPt[(h + 0) * SN + r] = (bf16_t)(v.x & 0xFFFFu);
+            sPt[(h + 1) * SN + r] = (bf16_t)(v.x >> 16);
+            sPt[(h + 2) * SN + r] = (bf16_t)(v.y & 0xFFFFu);
+            sPt[(h + 3) * SN + r] = (bf16_t)(v.y >> 16);
         }
-        for (int i = lane; i < 64 * K / 4; i += 64) {
-            const int r = (i * 4) / K;
+        for (int i = t; i < 64 * K / 4; i += 256) {
+            const int r = (4 * i) / K, c = (4 * i) % K;
             uint2 v = {0u, 0u};
             if (r < rows) v = reinterpret_cast<const uint2*>(X + (size_t)n0 * K)[i];
-            reinterpret_cast<uint2*>(sX)[i] = v;
+            sXt[(c + 0) * SN + r] = (bf16_t)(v.x & 0xFFFFu);
+            sXt[(c + 1) * SN + r] = (bf16_t)(v.x >> 16);
+            sXt[(c + 2) * SN + r] = (bf16_t)(v.y & 0xFFFFu);
+            sXt[(c + 3) * SN + r] = (bf16_t)(v.y >> 16);
         }
         __syncthreads();
 #pragma unroll
-        for (int s = 0; s < 64; s += 16) {
-            // A[h][n] = dP[n][h] (16 h x 16 n), B[n][col] = X[n][col]
-            bf16x4 a[4];
+        for (int s = 0; s < 64; s += 32) {
+            // A[h][n] = dP[n][h] (16 h x 32 n), B[n][col] = X[n][col] (32 n x 16 col)
+            const bf16x8 a = ld8(sPt + (wave * 16 + (lane & 15)) * SN + s + kq);
 #pragma unroll
-            for (int ht = 0; ht < 4; ht++) {
-                const int h = ht * 16 + (lane & 15);
-#pragma unroll
-                for (int j = 0; j < 4; j++) a[ht][j] = (short)sP[(s + kq + j) * 64 + h];
-            }
-#pragma unroll
-            for (int ct = 0; ct < CT; ct++) {
-                const int col = ct * 16 + (lane & 15);
-                bf16x4 b;
-#pragma unroll
-                for (int j = 0; j < 4; j++) b[j] = (short)sX[(s + kq + j) * K + col];
-#pragma unroll
-                for (int ht = 0; ht < 4; ht++)
-                    acc[ht][ct] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a[ht], b, acc[ht][ct], 0, 0, 0);
-            }
+            for (int ct = 0; ct < CT; ct++) acc[ct] = mfma32(a, ld8(sXt + (ct * 16 + (lane & 15)) * SN + s + kq), acc[ct]);
         }
     }
+    float* out = part + (size_t)blockIdx.x * 64 * K;
+    const int dq = 4 * (lane >> 4);
 #pragma unroll
-    for (int ht = 0; ht < 4; ht++)
+    for (int ct = 0; ct < CT; ct++) {
+        const int col = ct * 16 + (lane & 15);
 #pragma unroll
-        for (int ct = 0; ct < CT; ct++) {
-            const int col = ct * 16 + (lane & 15);
-#pragma unroll
-            for (int j = 0; j < 4; j++) {
-                const float v = acc[ht][ct][j];
-                if (v != 0.0f) atomicAdd(&gW[(ht * 16 + kq + j) * K + col], v);
-            }
+        for (int j = 0; j < 4; j++) out[(wave * 16 + dq + j) * K + col] = acc[ct][j];
+    }
+}
+
+// Fixed-order reductions of per-slot partials: out[i] (+)= the sum over slots of
+// part[slot * width + i].  Every gradient and the loss are summed this way, never with
+// floating-point atomics, so a gradient is the same bit for bit on every run.
+// Wide (weight matrices, few slots): one thread per element adds its slots in order.
+template <bool ACCUMULATE>
+__global__ void k_reduce_wide(const float* __restrict__ part, int slots, int width, float* __restrict__ out)
+{
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < width; i += gridDim.x * blockDim.x) {
+        float s = 0.f;
+        for (int k = 0; k < slots; k++) s += part[(size_t)k * width + i];
+        out[i] = ACCUMULATE ? out[i] + s : s;
+    }
+}
+
+// Narrow (biases, embeddings, the loss; thousands of slots): one workgroup per element,
+// each thread adds a fixed stride of slots, then a fixed binary tree in LDS.
+template <bool ACCUMULATE>
+__global__ __launch_bounds__(256) void k_reduce_tree(const float* __restrict__ part, int slots, int width,
+                                                     float* __restrict__ out)
+{
+    __shared__ float sh[256];
+    const int t = threadIdx.x;
+    for (int i = blockIdx.x; i < width; i += gridDim.x) {
+        float s = 0.f;
+        for (int k = t; k < slots; k += 256) s += part[(size_t)k * width + i];
+        sh[t] = s;
+        __syncthreads();
+        for (int o = 128; o > 0; o >>= 1) {
+            if (t < o) sh[t] += sh[t + o];
+            __syncthreads();
         }
+        if (t == 0) out[i] = ACCUMULATE ? out[i] + sh[0] : sh[0];
+        __syncthreads();
+    }
 }
 
 // ---------------------------------------------------------------------------------------
@@ -257,7 +294,7 @@ __global__ void k_build_z(int V, const float* __restrict__ Hl, bf16_t* __restric
 // Y = R W_y^T + b_y (W_y in BF16); with labels also the loss and dY.
 __global__ void k_head(int V, const float* __restrict__ Rv, const bf16_t* __restrict__ Wyq, const float* __restrict__ th,
                        float* __restrict__ Y, const float* __restrict__ target, const float* __restrict__ phase,
-                       const float* __restrict__ mask, float alpha, float* __restrict__ dY, float* __restrict__ loss)
+                       const float* __restrict__ mask, float alpha, float* __restrict__ dY, float* __restrict__ loss_part)
 {
     const int d = threadIdx.x & 63;
     float lsum = 0.f;
@@ -284,12 +321,15 @@ __global__ void k_head(int V, const float* __restrict__ Rv, const bf16_t* __rest
             }
         }
     }
-    if (target && d == 0 && lsum != 0.f) atomicAdd(loss, lsum);
+    // this wavefront's share of the loss (k_reduce sums the shares in order)
+    if (target && d == 0) loss_part[wave_global()] = lsum;
 }
 
-// dPo = (dY W_y) * [R > 0] -> BF16, plus gradients of W_y, b_y, b_o.
+// dPo = (dY W_y) * [R > 0] -> BF16, plus this wavefront's partial gradients of b_o, W_y
+// and b_y, laid out as the parameters are (b_o at O_BO, then W_y, then b_y).
+constexpr int HEAD_PART = O_BY + 2 - O_BO;
 __global__ void k_head_bwd(int V, const float* __restrict__ Rv, const float* __restrict__ dY,
-                           const bf16_t* __restrict__ Wyq, bf16_t* __restrict__ dPo, float* __restrict__ grad)
+                           const bf16_t* __restrict__ Wyq, bf16_t* __restrict__ dPo, float* __restrict__ part)
 {
     const int d = threadIdx.x & 63;
     const float w0 = from_bf16(Wyq[d]), w1 = from_bf16(Wyq[64 + d]);
@@ -307,12 +347,13 @@ __global__ void k_head_bwd(int V, const float* __restrict__ Rv, const float* __r
             gy1 += b;
         }
     }
-    atomicAdd(&grad[O_WY + d], gw0);
-    atomicAdd(&grad[O_WY + 64 + d], gw1);
-    atomicAdd(&grad[O_BO + d], gb);
+    float* o = part + (size_t)wave_global() * HEAD_PART;
+    o[d] = gb;
+    o[O_WY - O_BO + d] = gw0;
+    o[O_WY - O_BO + 64 + d] = gw1;
     if (d == 0) {
-        atomicAdd(&grad[O_BY], gy0);
-        atomicAdd(&grad[O_BY + 1], gy1);
+        o[O_BY - O_BO] = gy0;
+        o[O_BY - O_BO + 1] = gy1;
     }
 }
 
@@ -326,9 +367,9 @@ __global__ void k_bwd_z(int V, const float* __restrict__ dZ, float* __restrict__
     }
 }
 
-// dP = dH * [H > 0] -> BF16, and bias gradient.
+// dP = dH * [H > 0] -> BF16, and this wavefront's partial bias gradient.
 __global__ void k_bwd_mask(int N, const float* __restrict__ dH, const float* __restrict__ H, bf16_t* __restrict__ dP,
-                           float* __restrict__ gbias)
+                           float* __restrict__ part)
 {
     const int d = threadIdx.x & 63;
     float gb = 0.f;
@@ -338,14 +379,14 @@ __global__ void k_bwd_mask(int N, const float* __restrict__ dH, const float* __r
         gb += g;
         dP[i] = to_bf16(g);
     }
-    atomicAdd(&gbias[d], gb);
+    part[(size_t)wave_global() * HID + d] = gb;
 }
 
 // dH_c' = self part (from the next round, stride sstride) + 1/8 sum_{l in c} dXl[l][64 + d];
 // then masked by [H_c' > 0] into dPc (BF16) with the bias gradient.
 __global__ void k_bwd_clause(const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_edges, int C,
                              const float* __restrict__ self, int sstride, const float* __restrict__ dXl,
-                             const float* __restrict__ Hc1, bf16_t* __restrict__ dPc, float* __restrict__ gbias)
+                             const float* __restrict__ Hc1, bf16_t* __restrict__ dPc, float* __restrict__ part)
 {
     const int d = threadIdx.x & 63;
     float gb = 0.f;
@@ -358,7 +399,7 @@ __global__ void k_bwd_clause(const int32_t* __restrict__ c_off, const int32_t* _
         gb += g;
         dPc[(size_t)c * HID + d] = to_bf16(g);
     }
-    atomicAdd(&gbias[d], gb);
+    part[(size_t)wave_global() * HID + d] = gb;
 }
 
 // dH_l = dXl[l][d] + dXl[l^1][128 + d] + 1/4 sum_{c ni l} dXc[c][64 + d]
@@ -373,9 +414,10 @@ __global__ void k_bwd_lit(const int32_t* __restrict__ l_off, const int32_t* __re
     }
 }
 
-// gradients of the embeddings u, v from dH^0 (self part of dXc for clauses, stride 128)
+// this wavefront's partial gradients of the embeddings u_l, v_l, u_c, v_c (parameters
+// 0..255) from dH^0 (the self part of dXc for clauses, stride 128)
 __global__ void k_bwd_embed(const int32_t* __restrict__ l_off, const int32_t* __restrict__ c_off, int L, int C,
-                            const float* __restrict__ dHl, const float* __restrict__ dXc, float* __restrict__ grad)
+                            const float* __restrict__ dHl, const float* __restrict__ dXc, float* __restrict__ part)
 {
     const int d = threadIdx.x & 63;
     float gu = 0.f, gv = 0.f, cu = 0.f, cv = 0.f;
@@ -391,10 +433,11 @@ __global__ void k_bwd_embed(const int32_t* __restrict__ l_off, const int32_t* __
             cv += g;
         }
     }
-    atomicAdd(&grad[O_UL + d], gu);
-    atomicAdd(&grad[O_VL + d], gv);
-    atomicAdd(&grad[O_UC + d], cu);
-    atomicAdd(&grad[O_VC + d], cv);
+    float* o = part + (size_t)wave_global() * O_WC;
+    o[O_UL + d] = gu;
+    o[O_VL + d] = gv;
+    o[O_UC + d] = cu;
+    o[O_VC + d] = cv;
 }
 
 __global__ void k_quantize(const float* __restrict__ th, bf16_t* __restrict__ Wc, bf16_t* __restrict__ Wl,
@@ -429,12 +472,37 @@ __global__ void k_adam(float* __restrict__ th, const float* __restrict__ g, floa
 using namespace gpsat;
 using namespace gpsat::gnn;
 
+// Kernels that leave per-wavefront partials run on a fixed grid of RED_BLOCKS workgroups
+// of 4 wavefronts (one partial slot per wavefront; 16 wavefronts per CU to hide the gather
+// latency); the weight-gradient products on at most DW_BLOCKS workgroups of 4 wavefronts
+// (one partial matrix each).
+constexpr int RED_BLOCKS = 4096;
+constexpr int RED_WAVES = 4 * RED_BLOCKS;
+constexpr int DW_BLOCKS = 1024;
+constexpr size_t PART_FLOATS = std::max<size_t>((size_t)DW_BLOCKS * 64 * 192, (size_t)RED_WAVES * O_WC);
+
+// One batch's activations, carved from the model's arena (see gpsat_gnn_model).
+struct Acts {
+    std::vector<float*> Hl, Hc;
+    std::vector<bf16_t*> Xc, Xl;
+    bf16_t *Z = nullptr, *dP = nullptr;
+    float *Rv = nullptr, *dY = nullptr, *dXa = nullptr, *dXb = nullptr, *dHl = nullptr, *loss = nullptr;
+};
+
+// A model: FP32 master parameters, gradient, Adam moments, BF16 weight copies, the partials
+// of the gradient reductions, and a grow-only arena that holds the activations of the graph
+// being run.  Graphs own only their incidence structure and labels, so a training loop keeps
+// every batch's graph across steps without holding every batch's activations; calls on one
+// model therefore run one at a time.
 struct gpsat_gnn_model {
     float* th = nullptr;
     float* grad = nullptr;
     float* m = nullptr;
     float* v = nullptr;
+    float* part = nullptr;
     bf16_t *Wc = nullptr, *Wl = nullptr, *Wo = nullptr, *Wy = nullptr;
+    char* arena = nullptr;
+    size_t arena_bytes = 0;
     int64_t step = 0;
     bool dirty = true;
 };
@@ -443,11 +511,7 @@ struct gpsat_gnn_graph {
     int32_t V = 0, L = 0, C = 0, R = 3;
     int64_t E = 0;
     int32_t *c_off = nullptr, *c_edges = nullptr, *l_off = nullptr, *l_edges = nullptr;
-    std::vector<float*> Hl, Hc;
-    std::vector<bf16_t*> Xc, Xl;
-    bf16_t *Z = nullptr, *dP = nullptr;
-    float *Rv = nullptr, *Y = nullptr, *dY = nullptr, *dXa = nullptr, *dXb = nullptr, *dHl = nullptr, *loss = nullptr;
-    float *target = nullptr, *phase = nullptr, *mask = nullptr;
+    float *Y = nullptr, *target = nullptr, *phase = nullptr, *mask = nullptr;
     hipStream_t st = nullptr;
     std::vector<void*> owned;
     ~gpsat_gnn_graph()
@@ -484,7 +548,76 @@ int grid_rows(int64_t rows, int waves_per_block = 4)
     return (int)std::max<int64_t>(1, std::min<int64_t>(b, 65536));
 }
 
-hipError_t forward_pass(gpsat_gnn_model* m, gpsat_gnn_graph* g)
+// Lay a graph's activations out in `base` (nullptr: only count); returns the bytes used.
+// Every piece starts 256-byte aligned.
+size_t carve(const gpsat_gnn_graph* g, char* base, Acts& a)
+{
+    size_t off = 0;
+    auto take = [&](auto*& p, size_t count) {
+        using T = std::remove_reference_t<decltype(*p)>;
+        p = base ? (T*)(base + off) : nullptr;
+        off += (std::max<size_t>(count, 1) * sizeof(T) + 255) & ~(size_t)255;
+    };
+    const size_t L = g->L, C = g->C, V = g->V, R = g->R;
+    a.Hl.assign(R + 1, nullptr);
+    a.Hc.assign(R + 1, nullptr);
+    a.Xc.assign(R, nullptr);
+    a.Xl.assign(R, nullptr);
+    for (size_t t = 0; t <= R; t++) {
+        take(a.Hl[t], L * HID);
+        take(a.Hc[t], C * HID);
+        if (t < R) take(a.Xc[t], C * 128);
+        if (t < R) take(a.Xl[t], L * 192);
+    }
+    const size_t maxn = std::max({L, C, V});
+    take(a.Z, V * 128);
+    take(a.dP, maxn * HID);
+    take(a.Rv, V * HID);
+    take(a.dY, V * 2);
+    take(a.dXa, std::max(L * 192, V * 128));
+    take(a.dXb, C * 128);
+    take(a.dHl, L * HID);
+    take(a.loss, 1);
+    return off;
+}
+
+// The activations of graph g in model m's arena (grown if g needs more).
+hipError_t acts_for(gpsat_gnn_model* m, gpsat_gnn_graph* g, Acts& a)
+{
+    const size_t need = carve(g, nullptr, a);
+    if (need > m->arena_bytes) {
+        hipError_t e = hipDeviceSynchronize();
+        if (e != hipSuccess) return e;
+        (void)hipFree(m->arena);
+        m->arena = nullptr;
+        m->arena_bytes = 0;
+        e = hipMalloc(&m->arena, need);
+        if (e != hipSuccess) return e;
+        m->arena_bytes = need;
+    }
+    carve(g, m->arena, a);
+    return hipSuccess;
+}
+
+template <bool ACC>
+void reduce(const float* part, int slots, int width, float* out, hipStream_t st)
+{
+    if (width >= 4096)
+        hipLaunchKernelGGL(k_reduce_wide<ACC>, dim3((width + 255) / 256), dim3(256), 0, st, part, slots, width, out);
+    else
+        hipLaunchKernelGGL(k_reduce_tree<ACC>, dim3(width), dim3(256), 0, st, part, slots, width, out);
+}
+
+// gW += dP^T X over N nodes (weight gradient of one dense layer)
+template <int K>
+void dense_dw(gpsat_gnn_model* m, const bf16_t* dP, const bf16_t* X, int N, float* gW, hipStream_t st)
+{
+    const int nb = std::max(1, std::min(DW_BLOCKS, (N + 63) / 64));
+    hipLaunchKernelGGL((k_dense_bwd_dw<K>), dim3(nb), dim3(256), 0, st, dP, X, m->part, N);
+    reduce<true>(m->part, nb, 64 * K, gW, st);
+}
+
+hipError_t forward_pass(gpsat_gnn_model* m, gpsat_gnn_graph* g, const Acts& a)
 {
     hipStream_t st = g->st;
     if (m->dirty) {
@@ -493,60 +626,60 @@ hipError_t forward_pass(gpsat_gnn_model* m, gpsat_gnn_graph* g)
     }
     const int64_t rows = (int64_t)(g->L + g->C) * HID;
     hipLaunchKernelGGL(k_embed, dim3((int)std::min<int64_t>((rows + 255) / 256, 65536)), dim3(256), 0, st, g->l_off,
-                       g->c_off, g->L, g->C, m->th, g->Hl[0], g->Hc[0]);
+                       g->c_off, g->L, g->C, m->th, a.Hl[0], a.Hc[0]);
     for (int t = 0; t < g->R; t++) {
         hipLaunchKernelGGL(k_gather_clause, dim3(grid_rows(g->C)), dim3(256), 0, st, g->c_off, g->c_edges, g->C,
-                           g->Hl[t], g->Hc[t], g->Xc[t]);
-        hipLaunchKernelGGL((k_dense_fwd<128>), dim3((g->C + 63) / 64), dim3(256), 0, st, g->Xc[t], m->Wc,
-                           m->th + O_BC, g->Hc[t + 1], g->C);
-        hipLaunchKernelGGL(k_gather_lit, dim3(grid_rows(g->L)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, g->Hl[t],
-                           g->Hc[t + 1], g->Xl[t]);
-        hipLaunchKernelGGL((k_dense_fwd<192>), dim3((g->L + 63) / 64), dim3(256), 0, st, g->Xl[t], m->Wl,
-                           m->th + O_BL, g->Hl[t + 1], g->L);
+                           a.Hl[t], a.Hc[t], a.Xc[t]);
+        hipLaunchKernelGGL((k_dense_fwd<128>), dim3((g->C + 63) / 64), dim3(256), 0, st, a.Xc[t], m->Wc,
+                           m->th + O_BC, a.Hc[t + 1], g->C);
+        hipLaunchKernelGGL(k_gather_lit, dim3(grid_rows(g->L)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, a.Hl[t],
+                           a.Hc[t + 1], a.Xl[t]);
+        hipLaunchKernelGGL((k_dense_fwd<192>), dim3((g->L + 63) / 64), dim3(256), 0, st, a.Xl[t], m->Wl,
+                           m->th + O_BL, a.Hl[t + 1], g->L);
     }
-    hipLaunchKernelGGL(k_build_z, dim3(grid_rows(g->V)), dim3(256), 0, st, g->V, g->Hl[g->R], g->Z);
-    hipLaunchKernelGGL((k_dense_fwd<128>), dim3((g->V + 63) / 64), dim3(256), 0, st, g->Z, m->Wo, m->th + O_BO,
-                       g->Rv, g->V);
+    hipLaunchKernelGGL(k_build_z, dim3(grid_rows(g->V)), dim3(256), 0, st, g->V, a.Hl[g->R], a.Z);
+    hipLaunchKernelGGL((k_dense_fwd<128>), dim3((g->V + 63) / 64), dim3(256), 0, st, a.Z, m->Wo, m->th + O_BO, a.Rv,
+                       g->V);
     return hipGetLastError();
 }
 
-hipError_t head(gpsat_gnn_model* m, gpsat_gnn_graph* g, bool labels, float alpha)
+hipError_t head(gpsat_gnn_model* m, gpsat_gnn_graph* g, const Acts& a, bool labels, float alpha)
 {
-    hipLaunchKernelGGL(k_head, dim3(grid_rows(g->V)), dim3(256), 0, g->st, g->V, g->Rv, m->Wy, m->th, g->Y,
-                       labels ? g->target : nullptr, g->phase, g->mask, alpha, g->dY, g->loss);
+    hipLaunchKernelGGL(k_head, dim3(RED_BLOCKS), dim3(256), 0, g->st, g->V, a.Rv, m->Wy, m->th, g->Y,
+                       labels ? g->target : nullptr, g->phase, g->mask, alpha, a.dY, m->part);
+    if (labels) reduce<false>(m->part, RED_WAVES, 1, a.loss, g->st);
     return hipGetLastError();
 }
 
-hipError_t backward_pass(gpsat_gnn_model* m, gpsat_gnn_graph* g)
+hipError_t backward_pass(gpsat_gnn_model* m, gpsat_gnn_graph* g, const Acts& a)
 {
     hipStream_t st = g->st;
-    const int wblk = 256;
     // head and readout
-    hipLaunchKernelGGL(k_head_bwd, dim3(grid_rows(g->V)), dim3(256), 0, st, g->V, g->Rv, g->dY, m->Wy, g->dP, m->grad);
-    hipLaunchKernelGGL((k_dense_bwd_dw<128>), dim3(std::min(wblk, (g->V + 63) / 64)), dim3(64), 0, st, g->dP, g->Z,
-                       m->grad + O_WO, g->V);
-    hipLaunchKernelGGL((k_dense_bwd_dx<128>), dim3((g->V + 63) / 64), dim3(256), 0, st, g->dP, m->Wo, g->dXa, g->V);
-    hipLaunchKernelGGL(k_bwd_z, dim3(grid_rows(g->V)), dim3(256), 0, st, g->V, g->dXa, g->dHl);
+    hipLaunchKernelGGL(k_head_bwd, dim3(RED_BLOCKS), dim3(256), 0, st, g->V, a.Rv, a.dY, m->Wy, a.dP, m->part);
+    reduce<true>(m->part, RED_WAVES, HEAD_PART, m->grad + O_BO, st);
+    dense_dw<128>(m, a.dP, a.Z, g->V, m->grad + O_WO, st);
+    hipLaunchKernelGGL((k_dense_bwd_dx<128>), dim3((g->V + 63) / 64), dim3(256), 0, st, a.dP, m->Wo, a.dXa, g->V);
+    hipLaunchKernelGGL(k_bwd_z, dim3(grid_rows(g->V)), dim3(256), 0, st, g->V, a.dXa, a.dHl);
     // dXa holds dXl (L x 192), dXb holds dXc (C x 128)
     const float* self = nullptr;
     for (int t = g->R - 1; t >= 0; t--) {
-        hipLaunchKernelGGL(k_bwd_mask, dim3(grid_rows(g->L)), dim3(256), 0, st, g->L, g->dHl, g->Hl[t + 1], g->dP,
-                           m->grad + O_BL);
-        hipLaunchKernelGGL((k_dense_bwd_dw<192>), dim3(std::min(wblk, (g->L + 63) / 64)), dim3(64), 0, st, g->dP,
-                           g->Xl[t], m->grad + O_WL, g->L);
-        hipLaunchKernelGGL((k_dense_bwd_dx<192>), dim3((g->L + 63) / 64), dim3(256), 0, st, g->dP, m->Wl, g->dXa, g->L);
+        hipLaunchKernelGGL(k_bwd_mask, dim3(RED_BLOCKS), dim3(256), 0, st, g->L, a.dHl, a.Hl[t + 1], a.dP, m->part);
+        reduce<true>(m->part, RED_WAVES, HID, m->grad + O_BL, st);
+        dense_dw<192>(m, a.dP, a.Xl[t], g->L, m->grad + O_WL, st);
+        hipLaunchKernelGGL((k_dense_bwd_dx<192>), dim3((g->L + 63) / 64), dim3(256), 0, st, a.dP, m->Wl, a.dXa, g->L);
         // the self part of dH_c^{t+1} comes from round t+1's clause update (dXc of that round)
-        hipLaunchKernelGGL(k_bwd_clause, dim3(grid_rows(g->C)), dim3(256), 0, st, g->c_off, g->c_edges, g->C, self, 128,
-                           g->dXa, g->Hc[t + 1], g->dP, m->grad + O_BC);
-        hipLaunchKernelGGL((k_dense_bwd_dw<128>), dim3(std::min(wblk, (g->C + 63) / 64)), dim3(64), 0, st, g->dP,
-                           g->Xc[t], m->grad + O_WC, g->C);
-        hipLaunchKernelGGL((k_dense_bwd_dx<128>), dim3((g->C + 63) / 64), dim3(256), 0, st, g->dP, m->Wc, g->dXb, g->C);
-        hipLaunchKernelGGL(k_bwd_lit, dim3(grid_rows(g->L)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, g->dXa,
-                           g->dXb, g->dHl);
-        self = g->dXb;
+        hipLaunchKernelGGL(k_bwd_clause, dim3(RED_BLOCKS), dim3(256), 0, st, g->c_off, g->c_edges, g->C, self, 128,
+                           a.dXa, a.Hc[t + 1], a.dP, m->part);
+        reduce<true>(m->part, RED_WAVES, HID, m->grad + O_BC, st);
+        dense_dw<128>(m, a.dP, a.Xc[t], g->C, m->grad + O_WC, st);
+        hipLaunchKernelGGL((k_dense_bwd_dx<128>), dim3((g->C + 63) / 64), dim3(256), 0, st, a.dP, m->Wc, a.dXb, g->C);
+        hipLaunchKernelGGL(k_bwd_lit, dim3(grid_rows(g->L)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, a.dXa,
+                           a.dXb, a.dHl);
+        self = a.dXb;
     }
-    hipLaunchKernelGGL(k_bwd_embed, dim3(grid_rows(g->L + g->C)), dim3(256), 0, st, g->l_off, g->c_off, g->L, g->C,
-                       g->dHl, g->dXb, m->grad);
+    hipLaunchKernelGGL(k_bwd_embed, dim3(RED_BLOCKS), dim3(256), 0, st, g->l_off, g->c_off, g->L, g->C, a.dHl, a.dXb,
+                       m->part);
+    reduce<true>(m->part, RED_WAVES, O_WC, m->grad, st);
     return hipGetLastError();
 }
 
@@ -579,6 +712,7 @@ int gpsat_gnn_model_create(const float* theta, gpsat_gnn_model** out)
     gpsat_gnn_model* m = new gpsat_gnn_model();
     hipError_t e = hipMalloc(&m->th, 4 * N_PARAMS);
     if (e == hipSuccess) e = hipMalloc(&m->grad, 4 * N_PARAMS);
+    if (e == hipSuccess) e = hipMalloc(&m->part, 4 * PART_FLOATS);
     if (e == hipSuccess) e = hipMalloc(&m->m, 4 * N_PARAMS);
     if (e == hipSuccess) e = hipMalloc(&m->v, 4 * N_PARAMS);
     if (e == hipSuccess) e = hipMalloc(&m->Wc, 2 * 64 * 128);
@@ -601,8 +735,8 @@ int gpsat_gnn_model_destroy(gpsat_gnn_model* m)
 {
     if (!m) return GPSAT_OK;
     (void)hipDeviceSynchronize();
-    for (void* p : {(void*)m->th, (void*)m->grad, (void*)m->m, (void*)m->v, (void*)m->Wc, (void*)m->Wl, (void*)m->Wo,
-                    (void*)m->Wy})
+    for (void* p : {(void*)m->th, (void*)m->grad, (void*)m->m, (void*)m->v, (void*)m->part, (void*)m->arena,
+                    (void*)m->Wc, (void*)m->Wl, (void*)m->Wo, (void*)m->Wy})
         (void)hipFree(p);
     delete m;
     return GPSAT_OK;
@@ -656,30 +790,11 @@ int gpsat_gnn_graph_create(gpsat_batch* b, int32_t rounds, gpsat_gnn_graph** out
         for (int32_t e = c_off[c]; e < c_off[c + 1]; e++) l_edges[fill[c_edges[e]]++] = c;
     hipError_t e = hipStreamCreateWithFlags(&g->st, hipStreamNonBlocking);
     const size_t L = g->L, C = g->C, V = g->V;
-    g->Hl.resize(rounds + 1);
-    g->Hc.resize(rounds + 1);
-    g->Xc.resize(rounds);
-    g->Xl.resize(rounds);
     if (e == hipSuccess) e = g->alloc(g->c_off, C + 1);
     if (e == hipSuccess) e = g->alloc(g->c_edges, g->E);
     if (e == hipSuccess) e = g->alloc(g->l_off, L + 1);
     if (e == hipSuccess) e = g->alloc(g->l_edges, g->E);
-    for (int t = 0; t <= rounds && e == hipSuccess; t++) {
-        e = g->alloc(g->Hl[t], L * HID);
-        if (e == hipSuccess) e = g->alloc(g->Hc[t], C * HID);
-        if (e == hipSuccess && t < rounds) e = g->alloc(g->Xc[t], C * 128);
-        if (e == hipSuccess && t < rounds) e = g->alloc(g->Xl[t], L * 192);
-    }
-    const size_t maxn = std::max({L, C, V});
-    if (e == hipSuccess) e = g->alloc(g->Z, V * 128);
-    if (e == hipSuccess) e = g->alloc(g->dP, maxn * HID);
-    if (e == hipSuccess) e = g->alloc(g->Rv, V * HID);
     if (e == hipSuccess) e = g->alloc(g->Y, V * 2);
-    if (e == hipSuccess) e = g->alloc(g->dY, V * 2);
-    if (e == hipSuccess) e = g->alloc(g->dXa, std::max(L * 192, V * 128));
-    if (e == hipSuccess) e = g->alloc(g->dXb, C * 128);
-    if (e == hipSuccess) e = g->alloc(g->dHl, L * HID);
-    if (e == hipSuccess) e = g->alloc(g->loss, 1);
     if (e == hipSuccess) e = g->alloc(g->target, V);
     if (e == hipSuccess) e = g->alloc(g->phase, V);
     if (e == hipSuccess) e = g->alloc(g->mask, V);
@@ -721,8 +836,10 @@ int gpsat_gnn_forward(gpsat_gnn_model* m, gpsat_gnn_graph* g, float* y)
         set_error("null argument");
         return GPSAT_ERR_ARG;
     }
-    GNN_TRY(forward_pass(m, g));
-    GNN_TRY(head(m, g, false, 0.f));
+    Acts a;
+    GNN_TRY(acts_for(m, g, a));
+    GNN_TRY(forward_pass(m, g, a));
+    GNN_TRY(head(m, g, a, false, 0.f));
     GNN_TRY(hipMemcpyAsync(y, g->Y, 8 * (size_t)g->V, hipMemcpyDeviceToHost, g->st));
     GNN_TRY(hipStreamSynchronize(g->st));
     return GPSAT_OK;
@@ -734,12 +851,13 @@ int gpsat_gnn_loss_grad(gpsat_gnn_model* m, gpsat_gnn_graph* g, float alpha, flo
         set_error("null argument");
         return GPSAT_ERR_ARG;
     }
-    GNN_TRY(hipMemsetAsync(g->loss, 0, 4, g->st));
+    Acts a;
+    GNN_TRY(acts_for(m, g, a));
     GNN_TRY(hipMemsetAsync(m->grad, 0, 4 * N_PARAMS, g->st));
-    GNN_TRY(forward_pass(m, g));
-    GNN_TRY(head(m, g, true, alpha));
-    GNN_TRY(backward_pass(m, g));
-    if (loss) GNN_TRY(hipMemcpyAsync(loss, g->loss, 4, hipMemcpyDeviceToHost, g->st));
+    GNN_TRY(forward_pass(m, g, a));
+    GNN_TRY(head(m, g, a, true, alpha));
+    GNN_TRY(backward_pass(m, g, a));
+    if (loss) GNN_TRY(hipMemcpyAsync(loss, a.loss, 4, hipMemcpyDeviceToHost, g->st));
     if (grad) GNN_TRY(hipMemcpyAsync(grad, m->grad, 4 * N_PARAMS, hipMemcpyDeviceToHost, g->st));
     if (y) GNN_TRY(hipMemcpyAsync(y, g->Y, 8 * (size_t)g->V, hipMemcpyDeviceToHost, g->st));
     GNN_TRY(hipStreamSynchronize(g->st));
@@ -752,11 +870,12 @@ int gpsat_gnn_train_step(gpsat_gnn_model* m, gpsat_gnn_graph* g, float alpha, fl
         set_error("bad argument");
         return GPSAT_ERR_ARG;
     }
-    GNN_TRY(hipMemsetAsync(g->loss, 0, 4, g->st));
+    Acts a;
+    GNN_TRY(acts_for(m, g, a));
     GNN_TRY(hipMemsetAsync(m->grad, 0, 4 * N_PARAMS, g->st));
-    GNN_TRY(forward_pass(m, g));
-    GNN_TRY(head(m, g, true, alpha));
-    GNN_TRY(backward_pass(m, g));
+    GNN_TRY(forward_pass(m, g, a));
+    GNN_TRY(head(m, g, a, true, alpha));
+    GNN_TRY(backward_pass(m, g, a));
     m->step++;
     const float b1 = 0.9f, b2 = 0.999f;
     const float c1 = 1.f - powf(b1, (float)m->step), c2 = 1.f - powf(b2, (float)m->step);
@@ -764,7 +883,7 @@ int gpsat_gnn_train_step(gpsat_gnn_model* m, gpsat_gnn_graph* g, float alpha, fl
                        c1, c2);
     m->dirty = true;
     GNN_TRY(hipGetLastError());
-    if (loss) GNN_TRY(hipMemcpyAsync(loss, g->loss, 4, hipMemcpyDeviceToHost, g->st));
+    if (loss) GNN_TRY(hipMemcpyAsync(loss, a.loss, 4, hipMemcpyDeviceToHost, g->st));
     GNN_TRY(hipStreamSynchronize(g->st));
     return GPSAT_OK;
 }

@@ -165,25 +165,58 @@ def train(graphs: Sequence[GnnGraph], theta: Optional[np.ndarray] = None, epochs
 
 def train_sets(sets, theta: Optional[np.ndarray] = None, epochs: int = 10, lr: float = 2e-3, alpha: float = 1.0,
                log=None) -> np.ndarray:
-    """Adam over labelled batches [(GpuBatch, (target, phase, mask)), ...], building each
-    batch's graph (incidence structure + activations) only for its own step, so device
-    memory holds one batch's activations at a time however many batches there are."""
+    """Adam over labelled batches [(GpuBatch, (target, phase, mask)), ...].  Each batch's
+    graph (its incidence structure and labels) is built once and kept for every epoch; the
+    activations of a step live in the model's arena, so device memory holds one batch's
+    activations however many batches there are."""
     model = GnnModel(theta if theta is not None else init_params(0))
+    graphs = []
+    for gb, lab in sets:
+        g = GnnGraph(gb)
+        g.labels(*lab)
+        graphs.append(g)
     history = []
     for ep in range(epochs):
-        losses = []
-        for gb, lab in sets:
-            g = GnnGraph(gb)
-            g.labels(*lab)
-            losses.append(g.train_step(model, lr, alpha))
-            g.close()
+        losses = [g.train_step(model, lr, alpha) for g in graphs]
         history.append(float(np.mean(losses)))
         if log:
             log(f"epoch {ep}: loss {history[-1]:.5f}")
     out = model.params()
+    for g in graphs:
+        g.close()
     model.close()
     train_sets.history = history
     return out
+
+
+# Forward FLOPs of one instance as the survey counts them: 3 rounds x (2n + m) nodes x
+# 2 * 64 * 64 * 2; a training step (forward + backward) is three times that.
+def forward_flops(n: int, m: int, rounds: int = ROUNDS) -> float:
+    return rounds * (2 * n + m) * 2 * 64 * 64 * 2.0
+
+
+def time_train_steps(n_inst: int = 512, n: int = 300, m: int = 1278, steps: int = 20, seed: int = 5):
+    """Wall-clock time of training steps (forward, loss, backward, Adam) on a batch of
+    n_inst random 3-SAT instances of the learned-part evaluation shape, with random labels.
+    Returns (ms per step, algorithmic FLOP/s by forward_flops)."""
+    import time
+    from . import workloads as W
+    b = W.random_3sat_batch(n_inst, n, m, seed=seed)
+    rng = np.random.default_rng(seed)
+    V = n_inst * n
+    with GpuBatch(b) as gb:
+        model = GnnModel(init_params(seed))
+        g = GnnGraph(gb)
+        g.labels(rng.random(V), (rng.random(V) < 0.5).astype(np.float32), (rng.random(V) < 0.5).astype(np.float32))
+        for _ in range(3):
+            g.train_step(model, 1e-3)
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            g.train_step(model, 1e-3)
+        dt = (time.perf_counter() - t0) / steps
+        g.close()
+        model.close()
+    return dt * 1e3, 3 * n_inst * forward_flops(n, m) / dt
 
 
 def propose(model: GnnModel, gb: GpuBatch, rounds: int = ROUNDS):

@@ -145,7 +145,14 @@ def main():
     ap.add_argument("--weights", default=DEFAULT_WEIGHTS)
     ap.add_argument("--eval-only", action="store_true")
     ap.add_argument("--report", default=None)
+    ap.add_argument("--time-steps", type=int, default=0,
+                    help="only time this many training steps on 512 instances of n=300, m=1278 and exit")
     a = ap.parse_args()
+    if a.time_steps:
+        ms, flops = Lr.time_train_steps(steps=a.time_steps)
+        print(json.dumps({"train_step_ms": ms, "algorithmic_tflops": flops / 1e12,
+                          "frac_of_bf16_peak": flops / 2.5e15}))
+        return
     report = {}
     if a.eval_only:
         theta = np.load(a.weights)

@@ -13,7 +13,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "liboracle.so")
+# GPSAT_ORACLE_LIB selects another build of the oracle (the sanitizer build of
+# tests/run_sanitized.sh)
+_LIB_PATH = os.environ.get("GPSAT_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 SAT, UNSAT, UNDEF = 0, 1, 2

@@ -203,6 +203,44 @@ def test_3sat_n200_full_batch_properties(orc):
     np.testing.assert_array_equal(np.concatenate([out[base[i]:base[i + 1]] for i in sample]), e_out)
 
 
+@pytest.mark.parametrize("p", [0.3, 0.05])
+def test_timed_fused_kernel_full_size(orc, p):
+    """The benchmark's own launch, the fused register-resident kernel on 10,000 x (n=200,
+    m=852) with 64 rows through the device entry point, equals the separate evaluation and
+    BCP launches output for output over the whole batch, and the oracle on a 256-instance
+    sample (VariablesStateHandler.cu:180-206, WatchedClausesList.cu:46-309)."""
+    print(f"[eval/bcp] fused launch at full size, p={p}", flush=True)
+    N, n, m, A = 10_000, 200, 852, 64
+    b = W.random_3sat_batch(N, n, m, seed=0)
+    rows = W.random_assignments(b, A, p, seed=0)
+    with G.GpuBatch(b) as gb:
+        assert G.tune_get("eval_fast3") == 1
+        f_counts, f_out, f_conflict, f_props = fused(gb, rows, A)
+        rb = G.DevBuf.from_host(rows)
+        cb, ob = G.DevBuf(N * A * 12), G.DevBuf(rows.size * 4)
+        fb, pb = G.DevBuf(N * A * 4), G.DevBuf(N * A * 4)
+        gb.clause_eval_dev(rb, A, cb)
+        gb.bcp_fixpoint_dev(rb, A, ob, fb, pb)
+        gb.sync()
+        counts = cb.to_host(np.int32, N * A * 3).reshape(N, A, 3)
+        out = ob.to_host(np.uint32, rows.size)
+        conflict = fb.to_host(np.int32, N * A).reshape(N, A)
+        props = pb.to_host(np.int32, N * A).reshape(N, A)
+    np.testing.assert_array_equal(f_counts, counts)
+    np.testing.assert_array_equal(f_out, out)
+    np.testing.assert_array_equal(f_conflict, conflict)
+    np.testing.assert_array_equal(f_props, props)
+    sample = np.random.default_rng(2).choice(N, 256, replace=False)
+    sb = b.subset(sample)
+    base = b.row_base(A)
+    srows = np.concatenate([rows[base[i]:base[i + 1]] for i in sample])
+    np.testing.assert_array_equal(f_counts[sample], orc.eval_batch(sb, srows, A))
+    e_out, e_conf, e_props = orc.bcp_batch(sb, srows, A)
+    np.testing.assert_array_equal(f_conflict[sample], e_conf)
+    np.testing.assert_array_equal(f_props[sample], e_props)
+    np.testing.assert_array_equal(np.concatenate([f_out[base[i]:base[i + 1]] for i in sample]), e_out)
+
+
 @pytest.mark.parametrize("A", [1, 8, 64])
 def test_large_instances_hbm_mask_table(orc, A):
     """Instances whose 2n literal masks exceed a workgroup's LDS (n = 2,500 and 5,000,

@@ -46,10 +46,11 @@ def test_forward_matches_oracle():
     ref = R.forward(theta.astype(np.float64), g, quant=True)
     err = np.abs(y - ref).max()
     scale = np.abs(ref).max()
-    print("max abs err", err, "scale", scale)
+    rel = np.linalg.norm(y - ref) / np.linalg.norm(ref)
+    print("max abs err", err, "scale", scale, "relative L2", rel)
     # same BF16 rounding points; FP32 vs FP64 accumulation may flip an occasional rounding
+    assert rel <= 2e-3
     assert err <= 2e-2 * max(1.0, scale)
-    assert np.corrcoef(y[:, 0], ref[:, 0])[0, 1] > 0.9999
 
 
 def test_gradient_matches_oracle():
@@ -67,7 +68,7 @@ def test_gradient_matches_oracle():
         a, r = grad[sl], rgrad[sl]
         rel = np.linalg.norm(a - r) / max(np.linalg.norm(r), 1e-12)
         print(name, "rel err", rel)
-        assert rel <= 3e-2, (name, rel)
+        assert rel <= 5e-3, (name, rel)
 
 
 def test_training_lowers_loss():
