@@ -354,8 +354,9 @@ class GpuBatch:
     # ---- trajectory mode
     def solve_ref(self, dead=None, max_iterations: int = 0, learned_cap: int = 0, learned_lits: int = 0) -> dict:
         """The reference's own sequential search, one instance per lane (gpsat_solve_ref):
-        status (0 SAT, 1 UNSAT, 2 UNDEF, 4 capacity), model rows and per-instance stats
-        (decisions, conflicts, learned, iterations, restarts)."""
+        status (0 SAT, 1 UNSAT, 2 UNDEF, 4 a graph vertex past the reference's edge capacity),
+        model rows and per-instance stats (decisions, conflicts, learned, iterations,
+        restarts).  learned_cap / learned_lits are accepted and not used."""
         N = self.n_inst
         status = np.zeros(N, dtype=np.int32)
         rows = np.zeros(max(self.row_words(1), 1), dtype=np.uint32)

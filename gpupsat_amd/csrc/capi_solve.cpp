@@ -190,8 +190,6 @@ extern "C" int gpsat_solve_ref(gpsat_batch* b, const uint8_t* dead, int32_t max_
         set_error("negative capacity");
         return GPSAT_ERR_ARG;
     }
-    if (learned_cap == 0) learned_cap = 4096;
-    if (learned_lits == 0) learned_lits = 65536;
     DeviceBatch& d = b->d;
     const int32_t N = d.n_inst;
     if (N == 0) return GPSAT_OK;
@@ -200,17 +198,15 @@ extern "C" int gpsat_solve_ref(gpsat_batch* b, const uint8_t* dead, int32_t max_
     int64_t ws = 0;
     for (int32_t i = 0; i < N; i++) {
         const InstMeta& mi = d.h_meta[i];
-        const int32_t L = d.h_coff[mi.coff_base + mi.m];
-        if ((int64_t)mi.m + learned_cap > INT32_MAX / 2 || (int64_t)L + learned_lits > INT32_MAX / 2) {
-            set_error("instance too large for the trajectory mode");
-            return GPSAT_ERR_CAPACITY;
-        }
+        const int32_t* co = d.h_coff.data() + mi.coff_base;
+        int32_t largest = 0;
+        for (int32_t c = 0; c < mi.m; c++) largest = std::max(largest, co[c + 1] - co[c]);
+        // max_implication_per_var = max(largest clause, MIN_IMPLICATION_PER_VAR = 100)
+        // (src/SATSolver/main.cu:148, Configs.cuh:55); the graph holds that many + 1 edges per vertex
         rm[i].ws = ws;
-        rm[i].ccap = mi.m + learned_cap;
-        rm[i].lcap = L + learned_lits;
-        rm[i].wtot = solve_ref_wtot(mi.n, L);
-        rm[i].pad = 0;
-        ws += (solve_ref_ws_ints(mi.n, mi.m, L, rm[i].ccap, rm[i].lcap) + 15) & ~(int64_t)15;
+        rm[i].nw = mi.m + 21;
+        rm[i].ecap = std::max(largest, 100) + 1;
+        ws += (solve_ref_ws_ints(mi.n, mi.m, rm[i].ecap) + 15) & ~(int64_t)15;
         var_base[i + 1] = var_base[i] + mi.n;
     }
     hipStream_t st = d.stream;

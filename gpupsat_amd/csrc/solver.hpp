@@ -92,10 +92,8 @@ struct SolveArgs {
 // Trajectory mode (solve_ref.hip): one instance per lane, the reference's own search.
 struct RefMeta {
     int64_t ws;          // first int of the instance's workspace slice
-    int32_t ccap;        // clause slots (original + learned)
-    int32_t lcap;        // literal slots (original + learned)
-    int32_t wtot;        // watch-entry slots over all literals
-    int32_t pad;
+    int32_t nw;          // watch-record slots: clauses + the 20 learned + 1
+    int32_t ecap;        // backward edges per graph vertex (max implication per var + 1)
 };
 
 struct RefArgs {
@@ -113,8 +111,7 @@ struct RefArgs {
     int64_t* stats;              // [n_inst * 5]: decisions, conflicts, learned, iterations, restarts
 };
 
-int64_t solve_ref_ws_ints(int32_t n, int32_t m, int32_t L, int32_t ccap, int32_t lcap);
-int32_t solve_ref_wtot(int32_t n, int32_t L);
+int64_t solve_ref_ws_ints(int32_t n, int32_t m, int32_t ecap);
 hipError_t launch_solve_ref(const RefArgs& a, hipStream_t st);
 
 size_t solve_state_lds_bytes(int32_t max_n);

@@ -259,16 +259,21 @@ int gpsat_solve_noalloc(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t
                         int32_t *var_bumps, gpsat_solve_info *info);
 
 /* Trajectory mode: the reference's own sequential search per instance, step for step
- * (SATSolver::solve, src/SATSolver/SATSolver.cu:67-218 with the default strategies of
- * Configs.cuh: last-free-var decisions, decision-scheme learning, 20-clause round-robin
- * learned DB, geometric restarts 100 x 1.3, MAX_ITERATIONS), one instance per GPU lane.
- * Verdicts, iteration and conflict counts and the UNDEF set under max_iterations are the
- * reference scheme's.  dead: per var (instances back to back, may be NULL) variables the
- * preprocessing solved, kept out of the free list as in FormulaData/Results.  Capacities:
- * learned_cap learned clauses and learned_lits learned literals per instance (0: 4096 and
- * 65536); an instance that runs out ends with status 4 and nothing is written past them.
- * Outputs: status[n_inst] (0 SAT, 1 UNSAT, 2 UNDEF, 4 capacity), model_rows (literal
- * bitmaps, gpsat_batch_row_words(b, 1) words), stats[n_inst*5] (decisions, conflicts,
+ * (SATSolver::solve, src/SATSolver/SATSolver.cu:67-218, with the default strategies of
+ * Configs.cuh), one instance per GPU lane, over the reference's structures: per-variable watch
+ * lists with the fewest-watched-clauses choice and the blocked list
+ * (BCPStrategy/ClauseListStructure.cu:39-243), depth-first propagation of pending implications
+ * (WatchedClausesList.cu:46-309), the implication graph and decision-scheme learning
+ * (ConflictAnalysis/GraphAnalyzer.cu:31-138, GraphStructure.cu:79-330), Backtracker's backjump
+ * (Backtracker.cu:37-75), the 20-clause round-robin repository, geometric restarts 100 x 1.3
+ * and MAX_ITERATIONS (max_iterations > 0).  Clauses of fewer than two literals are not watched,
+ * as in the reference's solver (its input has been preprocessed).  dead: per var (instances
+ * back to back, may be NULL) variables the preprocessing solved, kept out of the free list as in
+ * FormulaData/Results.  learned_cap and learned_lits are not used (the search holds at most the
+ * reference's 20 learned clauses and the one being analysed); pass 0.  Outputs: status[n_inst]
+ * (0 SAT, 1 UNSAT, 2 UNDEF, 4 a graph vertex needed more backward edges than the reference's
+ * max_implication_per_var + 1, where the reference writes past its edge array), model_rows
+ * (literal bitmaps, gpsat_batch_row_words(b, 1) words), stats[n_inst*5] (decisions, conflicts,
  * learned clauses, iterations, restarts; any output but status may be NULL). */
 int gpsat_solve_ref(gpsat_batch *b, const uint8_t *dead, int32_t max_iterations, int32_t learned_cap,
                     int32_t learned_lits, int32_t *status, uint32_t *model_rows, int64_t *stats);

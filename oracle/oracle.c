@@ -247,56 +247,6 @@ int orc_preprocess(const int32_t *lits, const int64_t *clause_off, int32_t m, in
     return status;
 }
 
-/* ------------------------------------------------------------------------------------------
- * Shared solver scaffolding: clause arena (originals + append-only learned store), 2WL. */
-typedef struct {
-    int32_t *lits; int64_t nl, capl;       /* literal arena */
-    int64_t *start; int32_t *len; int32_t nc, capc;
-} arena_t;
-
-static void arena_init(arena_t *a, const int32_t *lits, const int64_t *clause_off, int32_t m)
-{
-    a->nl = clause_off[m] - clause_off[0];
-    a->capl = a->nl * 2 + 1024;
-    a->lits = (int32_t *)malloc(sizeof(int32_t) * a->capl);
-    memcpy(a->lits, lits + clause_off[0], sizeof(int32_t) * a->nl);
-    a->capc = m * 2 + 64;
-    a->start = (int64_t *)malloc(sizeof(int64_t) * a->capc);
-    a->len = (int32_t *)malloc(sizeof(int32_t) * a->capc);
-    a->nc = m;
-    for (int32_t c = 0; c < m; c++) {
-        a->start[c] = clause_off[c] - clause_off[0];
-        a->len[c] = (int32_t)(clause_off[c + 1] - clause_off[c]);
-    }
-}
-
-static int32_t arena_add(arena_t *a, const int32_t *l, int32_t n)
-{
-    if (a->nl + n > a->capl) {
-        while (a->nl + n > a->capl) a->capl *= 2;
-        a->lits = (int32_t *)realloc(a->lits, sizeof(int32_t) * a->capl);
-    }
-    if (a->nc + 1 > a->capc) {
-        a->capc *= 2;
-        a->start = (int64_t *)realloc(a->start, sizeof(int64_t) * a->capc);
-        a->len = (int32_t *)realloc(a->len, sizeof(int32_t) * a->capc);
-    }
-    memcpy(a->lits + a->nl, l, sizeof(int32_t) * n);
-    a->start[a->nc] = a->nl;
-    a->len[a->nc] = n;
-    a->nl += n;
-    return a->nc++;
-}
-
-static void arena_free(arena_t *a) { free(a->lits); free(a->start); free(a->len); }
-
-typedef struct { int32_t *d; int32_t n, cap; } ivec;
-static inline void iv_push(ivec *v, int32_t x)
-{
-    if (v->n == v->cap) { v->cap = v->cap ? v->cap * 2 : 8; v->d = (int32_t *)realloc(v->d, sizeof(int32_t) * v->cap); }
-    v->d[v->n++] = x;
-}
-
 /* value of literal l given var values val[] in {-1,0,1}: 1 true, 0 false, -1 undef */
 static inline int lit_value(const int8_t *val, int32_t l)
 {
@@ -305,258 +255,7 @@ static inline int lit_value(const int8_t *val, int32_t l)
     return (l & 1) ? x : !x;
 }
 
-/* ------------------------------------------------------------------------------------------
- * a6..a13: the reference's sequential scheme. */
-typedef struct {
-    int32_t n;
-    arena_t ar;
-    int8_t *val; int32_t *level; int32_t *reason; uint8_t *active;
-    int32_t *trail; int32_t ntrail; int32_t qhead; int32_t dl;
-    uint8_t *is_decision;
-    ivec *watch;            /* per literal: clauses watching it */
-    int32_t *w0, *w1;       /* watched literals per clause */
-    /* ordered free-var list (doubly linked): VariablesStateHandler free_vars semantics */
-    int32_t *fprev, *fnext; int32_t fhead, ftail; uint8_t *infree; int32_t nfree;
-} rsolver;
-
-static void fl_append(rsolver *s, int32_t v)
-{
-    if (s->infree[v]) return;
-    s->infree[v] = 1; s->nfree++;
-    s->fprev[v] = s->ftail; s->fnext[v] = -1;
-    if (s->ftail >= 0) s->fnext[s->ftail] = v; else s->fhead = v;
-    s->ftail = v;
-}
-static void fl_remove(rsolver *s, int32_t v)
-{
-    if (!s->infree[v]) return;
-    s->infree[v] = 0; s->nfree--;
-    if (s->fprev[v] >= 0) s->fnext[s->fprev[v]] = s->fnext[v]; else s->fhead = s->fnext[v];
-    if (s->fnext[v] >= 0) s->fprev[s->fnext[v]] = s->fprev[v]; else s->ftail = s->fprev[v];
-}
-
-static void rs_assign(rsolver *s, int32_t l, int32_t lvl, int32_t reason, int decision)
-{
-    int32_t v = LIT_VAR(l);
-    s->val[v] = (int8_t)(l & 1);
-    s->level[v] = lvl;
-    s->reason[v] = reason;
-    s->is_decision[v] = (uint8_t)decision;
-    s->trail[s->ntrail++] = l;
-    fl_remove(s, v);
-}
-
-static void rs_watch(rsolver *s, int32_t c)
-{
-    const int32_t *cl = s->ar.lits + s->ar.start[c];
-    s->w0[c] = cl[0]; s->w1[c] = cl[1];
-    iv_push(&s->watch[cl[0]], c);
-    iv_push(&s->watch[cl[1]], c);
-}
-
-/* Watched-literal propagation to a fixpoint.  Returns the conflicting clause or -1. */
-static int32_t rs_propagate(rsolver *s)
-{
-    while (s->qhead < s->ntrail) {
-        int32_t p = s->trail[s->qhead++];
-        int32_t fl = LIT_NEG(p);          /* literal that became false */
-        ivec *ws = &s->watch[fl];
-        int32_t i = 0, j = 0, confl = -1;
-        while (i < ws->n) {
-            int32_t c = ws->d[i++];
-            if (!s->active[c] || (s->w0[c] != fl && s->w1[c] != fl)) continue; /* stale */
-            int32_t other = (s->w0[c] == fl) ? s->w1[c] : s->w0[c];
-            if (confl >= 0 || lit_value(s->val, other) == 1) { ws->d[j++] = c; continue; }
-            const int32_t *cl = s->ar.lits + s->ar.start[c];
-            int32_t len = s->ar.len[c], k, moved = 0;
-            for (k = 0; k < len; k++) {
-                int32_t q = cl[k];
-                if (q == s->w0[c] || q == s->w1[c]) continue;
-                if (lit_value(s->val, q) != 0) {
-                    if (s->w0[c] == fl) s->w0[c] = q; else s->w1[c] = q;
-                    iv_push(&s->watch[q], c);
-                    moved = 1;
-                    break;
-                }
-            }
-            if (moved) continue;
-            ws->d[j++] = c;
-            if (lit_value(s->val, other) == 0) confl = c;
-            else rs_assign(s, other, s->dl, c, 0);
-        }
-        ws->n = j;
-        if (confl >= 0) return confl;
-    }
-    return -1;
-}
-
-/* VariablesStateHandler::backtrack_to (VariablesStateHandler.cu:367-399): implications
- * above the level are undone first (in implication order), then decisions; each undone
- * var is appended to the free list. */
-static void rs_backtrack(rsolver *s, int32_t lvl)
-{
-    int32_t keep = 0;
-    for (int32_t i = 0; i < s->ntrail; i++) {
-        int32_t v = LIT_VAR(s->trail[i]);
-        if (s->level[v] > lvl && !s->is_decision[v]) fl_append(s, v);
-    }
-    for (int32_t i = 0; i < s->ntrail; i++) {
-        int32_t v = LIT_VAR(s->trail[i]);
-        if (s->level[v] > lvl && s->is_decision[v]) fl_append(s, v);
-    }
-    for (int32_t i = 0; i < s->ntrail; i++) {
-        int32_t l = s->trail[i], v = LIT_VAR(l);
-        if (s->level[v] > lvl) { s->val[v] = -1; s->reason[v] = -1; }
-        else s->trail[keep++] = l;
-    }
-    s->ntrail = keep;
-    s->qhead = keep;
-    s->dl = lvl;
-}
-
-int orc_solve_ref(const int32_t *lits, const int64_t *clause_off, int32_t m, int32_t n,
-                  const uint8_t *dead, int32_t max_iterations, int8_t *model, int64_t *stats)
-{
-    rsolver S, *s = &S;
-    memset(s, 0, sizeof(*s));
-    s->n = n;
-    arena_init(&s->ar, lits, clause_off, m);
-    int32_t nv = n ? n : 1;
-    s->val = (int8_t *)malloc(nv); memset(s->val, -1, nv);
-    s->level = (int32_t *)calloc(nv, sizeof(int32_t));
-    s->reason = (int32_t *)malloc(sizeof(int32_t) * nv);
-    s->is_decision = (uint8_t *)calloc(nv, 1);
-    s->trail = (int32_t *)malloc(sizeof(int32_t) * nv);
-    s->watch = (ivec *)calloc(2 * nv, sizeof(ivec));
-    s->fprev = (int32_t *)malloc(sizeof(int32_t) * nv);
-    s->fnext = (int32_t *)malloc(sizeof(int32_t) * nv);
-    s->infree = (uint8_t *)calloc(nv, 1);
-    s->fhead = s->ftail = -1;
-    int64_t st_dec = 0, st_conf = 0, st_learn = 0, st_iter = 0, st_rest = 0;
-    for (int32_t v = 0; v < n; v++) { s->reason[v] = -1; if (!(dead && dead[v])) fl_append(s, v); }
-    int32_t capc = m + 64;
-    s->active = (uint8_t *)calloc(capc, 1);
-    s->w0 = (int32_t *)malloc(sizeof(int32_t) * capc);
-    s->w1 = (int32_t *)malloc(sizeof(int32_t) * capc);
-    int status = ORC_UNDEF;
-    int32_t largest = 0;
-    for (int32_t c = 0; c < m; c++) {
-        int32_t len = s->ar.len[c];
-        if (len > largest) largest = len;
-        s->active[c] = 1;
-        if (len == 0) status = ORC_UNSAT;
-        else if (len == 1) {
-            int32_t l = s->ar.lits[s->ar.start[c]];
-            int lv = lit_value(s->val, l);
-            if (lv == 0) status = ORC_UNSAT;
-            else if (lv < 0) rs_assign(s, l, 0, -1, 0);
-        } else rs_watch(s, c);
-    }
-    /* max_implication_per_var = max(largest clause, MIN_IMPLICATION_PER_VAR) (main.cu:148) */
-    int32_t max_impl = largest > 100 ? largest : 100;
-    int32_t db[20], db_n = 0, db_pos = 0;
-    int32_t restart_thr = 100, restart_cnt = 0;
-    float restart_factor = 1.3f;
-    int32_t *learnt = (int32_t *)malloc(sizeof(int32_t) * (nv + 1));
-    uint8_t *seen = (uint8_t *)calloc(nv, 1);
-    int32_t *stack = (int32_t *)malloc(sizeof(int32_t) * (nv + 1));
-    if (status == ORC_UNDEF && rs_propagate(s) >= 0) status = ORC_UNSAT;
-    if (status == ORC_UNDEF && s->nfree == 0) status = ORC_SAT;
-    int32_t iterations = 0;
-    while (status == ORC_UNDEF) {
-        /* DecisionMaker::decide: last free var, positive polarity, level + 1 */
-        int32_t v = s->ftail;
-        s->dl++;
-        st_dec++;
-        rs_assign(s, 2 * v + 1, s->dl, -1, 1);
-        int32_t n_conf = 0;
-        int32_t confl;
-        while ((confl = rs_propagate(s)) >= 0) {
-            n_conf++; st_conf++;
-            if (s->dl <= 0) { status = ORC_UNSAT; break; }
-            /* analyze_graph: DFS back from the conflict through same-level reasons; lower-level
-             * vars and the current-level root are collected (GraphAnalyzer.cu:31-138). */
-            int32_t nl = 0, sp = 0, cur = s->dl;
-            int32_t hi_lit = -1, second = -2;
-            memset(seen, 0, nv);
-            stack[sp++] = confl;
-            while (sp > 0) {
-                int32_t c = stack[--sp];
-                const int32_t *cl = s->ar.lits + s->ar.start[c];
-                int32_t len = s->ar.len[c];
-                for (int32_t k = 0; k < len; k++) {
-                    int32_t u = LIT_VAR(cl[k]);
-                    if (seen[u]) continue;
-                    seen[u] = 1;
-                    int32_t assigned = 2 * u + s->val[u];
-                    if (s->level[u] < cur || s->reason[u] < 0) {
-                        learnt[nl++] = LIT_NEG(assigned);
-                        if (s->level[u] >= cur) hi_lit = LIT_NEG(assigned);
-                        else if (s->level[u] > second) second = s->level[u];
-                    } else {
-                        stack[sp++] = s->reason[u];
-                    }
-                }
-            }
-            st_learn++;
-            /* implicate_and_backtrack_from_learnt_clause (ConflictAnalyzer.cu:105-151) */
-            if (nl == 1) {
-                rs_backtrack(s, 0);
-                rs_assign(s, learnt[0], 0, -1, 0);
-            } else {
-                if (second < 0) second = 0;
-                int32_t cid = arena_add(&s->ar, learnt, nl);
-                if (cid >= capc) {
-                    capc = cid * 2 + 64;
-                    s->active = (uint8_t *)realloc(s->active, capc);
-                    s->w0 = (int32_t *)realloc(s->w0, sizeof(int32_t) * capc);
-                    s->w1 = (int32_t *)realloc(s->w1, sizeof(int32_t) * capc);
-                }
-                s->active[cid] = 0;
-                rs_backtrack(s, second);
-                rs_assign(s, hi_lit, s->dl, cid, 0);
-                /* LearntClausesManager::learn_clause: 2 <= size < max_impl, 20-slot round robin
-                 * (ConflictAnalyzer.cu:153-171, LearntClauseRepository.cu:15-33) */
-                if (nl < max_impl) {
-                    /* watch the asserted literal and a literal of the backjump level */
-                    int32_t *cl = s->ar.lits + s->ar.start[cid];
-                    int32_t bi = 0, best = -1;
-                    for (int32_t k = 0; k < nl; k++) if (cl[k] == hi_lit) { int32_t t = cl[0]; cl[0] = cl[k]; cl[k] = t; }
-                    for (int32_t k = 1; k < nl; k++) {
-                        int32_t lv = s->level[LIT_VAR(cl[k])];
-                        if (lv > best) { best = lv; bi = k; }
-                    }
-                    { int32_t t = cl[1]; cl[1] = cl[bi]; cl[bi] = t; }
-                    if (db_n < 20) db[db_n++] = cid;
-                    else { s->active[db[db_pos]] = 0; db[db_pos] = cid; if (++db_pos >= db_n) db_pos = 0; }
-                    s->active[cid] = 1;
-                    rs_watch(s, cid);
-                }
-            }
-        }
-        if (status != ORC_UNDEF) break;
-        if (s->nfree == 0) { status = ORC_SAT; break; }
-        /* GeometricRestartsManager (GeometricRestartsManager.cu:3-31), SATSolver.cu:170-178 */
-        restart_cnt += n_conf;
-        if (restart_cnt >= restart_thr) {
-            restart_cnt = 0;
-            restart_thr = (int32_t)(restart_thr * restart_factor);
-            rs_backtrack(s, 0);
-            st_rest++;
-        }
-        iterations++;
-        st_iter = iterations;
-        if (max_iterations > 0 && iterations > max_iterations) { status = ORC_UNDEF; break; }
-    }
-    if (model) for (int32_t v = 0; v < n; v++) model[v] = (status == ORC_SAT) ? s->val[v] : -1;
-    if (stats) { stats[0] = st_dec; stats[1] = st_conf; stats[2] = st_learn; stats[3] = st_iter; stats[4] = st_rest; }
-    for (int32_t l = 0; l < 2 * nv; l++) free(s->watch[l].d);
-    free(s->watch); free(s->val); free(s->level); free(s->reason); free(s->is_decision);
-    free(s->trail); free(s->fprev); free(s->fnext); free(s->infree); free(s->active);
-    free(s->w0); free(s->w1); free(learnt); free(seen); free(stack);
-    arena_free(&s->ar);
-    return status;
-}
+/* a6..a13, the reference's sequential solver, is restated in refsolve.c. */
 
 /* ------------------------------------------------------------------------------------------
  * Independent verdict checkers (no reference counterpart). */

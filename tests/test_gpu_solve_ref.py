@@ -1,7 +1,7 @@
 """Trajectory mode (gpsat_solve_ref): the reference's sequential search on the device gives
-the oracle restatement's status, model and every counter -- decisions, conflicts, learned
-clauses, iterations, restarts -- instance by instance, including which instances the
-MAX_ITERATIONS cap turns into UNDEF."""
+the oracle restatement's (oracle/refsolve.c) status, model and every counter -- decisions,
+conflicts, learned clauses, iterations, restarts -- instance by instance, including which
+instances the MAX_ITERATIONS cap turns into UNDEF."""
 import glob
 import os
 
@@ -71,3 +71,14 @@ def test_fixtures_units_and_dead_vars(orc):
         dead.append(d)
     assert len(pf) >= 10
     check(orc, pf, dead=dead)
+
+
+def test_structured_and_mixed(orc):
+    """Colourings, PHP(7,6) (thousands of conflicts: the 20-slot learned repository wraps
+    around many times), and mixed k-SAT under a cap."""
+    print("[solve_ref] colouring, PHP(7,6), mixed k-SAT", flush=True)
+    fs = [W.coloring_batch(6, n_nodes=30, seed=1).instance(i) for i in range(6)]
+    fs.append(W.pigeonhole_formula(7, 6))
+    b, _ = W.mixed_ksat_batch(8, seed=2)
+    fs += [b.instance(i) for i in range(8) if b.instance(i).n_vars <= 200]
+    check(orc, fs, max_iterations=400)
