@@ -208,6 +208,30 @@ __global__ void k_reduce_wide(const float* __restrict__ part, int slots, int wid
     }
 }
 
+// Wide with many slots (weight-gradient partials of up to 1024 workgroups): first each
+// of `groups` workgroup rows adds its own contiguous run of slots, four columns per
+// thread, into part2[group * width + i]; k_reduce_wide then adds the groups in order.
+__global__ __launch_bounds__(256) void k_reduce_wide_split(const float* __restrict__ part, int slots, int width,
+                                                           float* __restrict__ part2)
+{
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;  // column quad
+    if (4 * q >= width) return;
+    const int per = (slots + gridDim.y - 1) / gridDim.y;
+    const int k0 = blockIdx.y * per, k1 = min(slots, k0 + per);
+    const float4* p = reinterpret_cast<const float4*>(part) + q;
+    const size_t stride = (size_t)width / 4;
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+#pragma unroll 8
+    for (int k = k0; k < k1; k++) {
+        const float4 v = p[(size_t)k * stride];
+        s.x += v.x;
+        s.y += v.y;
+        s.z += v.z;
+        s.w += v.w;
+    }
+    reinterpret_cast<float4*>(part2)[(size_t)blockIdx.y * stride + q] = s;
+}
+
 // Narrow (biases, embeddings, the loss; thousands of slots): one workgroup per element,
 // each thread adds a fixed stride of slots, then a fixed binary tree in LDS.
 template <bool ACCUMULATE>
@@ -254,16 +278,60 @@ __global__ void k_embed(const int32_t* __restrict__ l_off, const int32_t* __rest
 __device__ __forceinline__ int wave_global() { return (int)((blockIdx.x * blockDim.x + threadIdx.x) >> 6); }
 __device__ __forceinline__ int wave_count() { return (int)((gridDim.x * blockDim.x) >> 6); }
 
+// The gather kernels give each wavefront 4 rows at once, 16 lanes per row, each lane on 4
+// consecutive dimensions: a 256-byte row is one float4 load per lane, and a wavefront
+// keeps 4 rows' dependent offset -> edge -> neighbour loads in flight together.
+__device__ __forceinline__ int row_slot() { return (threadIdx.x & 63) >> 4; }
+__device__ __forceinline__ int col4() { return 4 * (threadIdx.x & 15); }
+__device__ __forceinline__ float4 ld4(const float* p) { return *reinterpret_cast<const float4*>(p); }
+__device__ __forceinline__ void add4(float4& s, const float4& x)
+{
+    s.x += x.x;
+    s.y += x.y;
+    s.z += x.z;
+    s.w += x.w;
+}
+__device__ __forceinline__ float4 mul4(float a, const float4& x) { return make_float4(a * x.x, a * x.y, a * x.z, a * x.w); }
+__device__ __forceinline__ void st_bf16x4(bf16_t* p, const float4& v)
+{
+    uint2 w;
+    w.x = (uint32_t)to_bf16(v.x) | (uint32_t)to_bf16(v.y) << 16;
+    w.y = (uint32_t)to_bf16(v.z) | (uint32_t)to_bf16(v.w) << 16;
+    *reinterpret_cast<uint2*>(p) = w;
+}
+
+// Sum over edges e0 .. e1-1 of X[edges[e] * stride + col .. col + 3], four neighbour rows in
+// flight at a time (a slot past e1 re-reads edge e0 of its quad and is not added), added in
+// edge order.
+__device__ __forceinline__ float4 gather_sum(const int32_t* __restrict__ edges, int e0, int e1,
+                                             const float* __restrict__ X, int stride, int col)
+{
+    float4 s = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int e = e0; e < e1; e += 4) {
+        const int k = e1 - e;
+        const int i0 = edges[e], i1 = edges[k > 1 ? e + 1 : e], i2 = edges[k > 2 ? e + 2 : e],
+                  i3 = edges[k > 3 ? e + 3 : e];
+        const float4 x0 = ld4(X + (size_t)i0 * stride + col), x1 = ld4(X + (size_t)i1 * stride + col);
+        const float4 x2 = ld4(X + (size_t)i2 * stride + col), x3 = ld4(X + (size_t)i3 * stride + col);
+        add4(s, x0);
+        if (k > 1) add4(s, x1);
+        if (k > 2) add4(s, x2);
+        if (k > 3) add4(s, x3);
+    }
+    return s;
+}
+
 // Xc = bf16([Hc | 1/4 sum_{l in c} Hl])
 __global__ void k_gather_clause(const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_edges, int C,
                                 const float* __restrict__ Hl, const float* __restrict__ Hc, bf16_t* __restrict__ Xc)
 {
-    const int d = threadIdx.x & 63;
-    for (int c = wave_global(); c < C; c += wave_count()) {
-        float s = 0.f;
-        for (int e = c_off[c]; e < c_off[c + 1]; e++) s += Hl[(size_t)c_edges[e] * HID + d];
-        Xc[(size_t)c * 128 + d] = to_bf16(Hc[(size_t)c * HID + d]);
-        Xc[(size_t)c * 128 + 64 + d] = to_bf16(0.25f * s);
+    const int d = col4();
+    for (int c0 = 4 * wave_global(); c0 < C; c0 += 4 * wave_count()) {
+        const int c = c0 + row_slot();
+        if (c >= C) continue;
+        const float4 s = gather_sum(c_edges, c_off[c], c_off[c + 1], Hl, HID, d);
+        st_bf16x4(Xc + (size_t)c * 128 + d, ld4(Hc + (size_t)c * HID + d));
+        st_bf16x4(Xc + (size_t)c * 128 + 64 + d, mul4(0.25f, s));
     }
 }
 
@@ -271,13 +339,14 @@ __global__ void k_gather_clause(const int32_t* __restrict__ c_off, const int32_t
 __global__ void k_gather_lit(const int32_t* __restrict__ l_off, const int32_t* __restrict__ l_edges, int L,
                              const float* __restrict__ Hl, const float* __restrict__ Hc1, bf16_t* __restrict__ Xl)
 {
-    const int d = threadIdx.x & 63;
-    for (int l = wave_global(); l < L; l += wave_count()) {
-        float s = 0.f;
-        for (int e = l_off[l]; e < l_off[l + 1]; e++) s += Hc1[(size_t)l_edges[e] * HID + d];
-        Xl[(size_t)l * 192 + d] = to_bf16(Hl[(size_t)l * HID + d]);
-        Xl[(size_t)l * 192 + 64 + d] = to_bf16(0.125f * s);
-        Xl[(size_t)l * 192 + 128 + d] = to_bf16(Hl[(size_t)(l ^ 1) * HID + d]);
+    const int d = col4();
+    for (int l0 = 4 * wave_global(); l0 < L; l0 += 4 * wave_count()) {
+        const int l = l0 + row_slot();
+        if (l >= L) continue;
+        const float4 s = gather_sum(l_edges, l_off[l], l_off[l + 1], Hc1, HID, d);
+        st_bf16x4(Xl + (size_t)l * 192 + d, ld4(Hl + (size_t)l * HID + d));
+        st_bf16x4(Xl + (size_t)l * 192 + 64 + d, mul4(0.125f, s));
+        st_bf16x4(Xl + (size_t)l * 192 + 128 + d, ld4(Hl + (size_t)(l ^ 1) * HID + d));
     }
 }
 
@@ -383,34 +452,48 @@ __global__ void k_bwd_mask(int N, const float* __restrict__ dH, const float* __r
 }
 
 // dH_c' = self part (from the next round, stride sstride) + 1/8 sum_{l in c} dXl[l][64 + d];
-// then masked by [H_c' > 0] into dPc (BF16) with the bias gradient.
+// then masked by [H_c' > 0] into dPc (BF16) with the bias gradient.  Row layout as in the
+// gather kernels; the 4 row slots' bias partials are added (slot 0+1, 2+3, then both)
+// before the wavefront writes its partial.
 __global__ void k_bwd_clause(const int32_t* __restrict__ c_off, const int32_t* __restrict__ c_edges, int C,
                              const float* __restrict__ self, int sstride, const float* __restrict__ dXl,
                              const float* __restrict__ Hc1, bf16_t* __restrict__ dPc, float* __restrict__ part)
 {
-    const int d = threadIdx.x & 63;
-    float gb = 0.f;
-    for (int c = wave_global(); c < C; c += wave_count()) {
-        float s = self ? self[(size_t)c * sstride + d] : 0.f;
-        float m = 0.f;
-        for (int e = c_off[c]; e < c_off[c + 1]; e++) m += dXl[(size_t)c_edges[e] * 192 + 64 + d];
-        s += 0.125f * m;
-        const float g = Hc1[(size_t)c * HID + d] > 0.f ? s : 0.f;
-        gb += g;
-        dPc[(size_t)c * HID + d] = to_bf16(g);
+    const int d = col4();
+    float4 gb = make_float4(0.f, 0.f, 0.f, 0.f);
+    for (int c0 = 4 * wave_global(); c0 < C; c0 += 4 * wave_count()) {
+        const int c = c0 + row_slot();
+        if (c >= C) continue;
+        float4 s = self ? ld4(self + (size_t)c * sstride + d) : make_float4(0.f, 0.f, 0.f, 0.f);
+        add4(s, mul4(0.125f, gather_sum(c_edges, c_off[c], c_off[c + 1], dXl, 192, 64 + d)));
+        const float4 h = ld4(Hc1 + (size_t)c * HID + d);
+        const float4 g = make_float4(h.x > 0.f ? s.x : 0.f, h.y > 0.f ? s.y : 0.f, h.z > 0.f ? s.z : 0.f,
+                                     h.w > 0.f ? s.w : 0.f);
+        add4(gb, g);
+        st_bf16x4(dPc + (size_t)c * HID + d, g);
     }
-    part[(size_t)wave_global() * HID + d] = gb;
+    for (int o = 16; o <= 32; o <<= 1) {
+        gb.x += __shfl_xor(gb.x, o);
+        gb.y += __shfl_xor(gb.y, o);
+        gb.z += __shfl_xor(gb.z, o);
+        gb.w += __shfl_xor(gb.w, o);
+    }
+    if (row_slot() == 0) *reinterpret_cast<float4*>(part + (size_t)wave_global() * HID + d) = gb;
 }
 
 // dH_l = dXl[l][d] + dXl[l^1][128 + d] + 1/4 sum_{c ni l} dXc[c][64 + d]
 __global__ void k_bwd_lit(const int32_t* __restrict__ l_off, const int32_t* __restrict__ l_edges, int L,
                           const float* __restrict__ dXl, const float* __restrict__ dXc, float* __restrict__ dHl)
 {
-    const int d = threadIdx.x & 63;
-    for (int l = wave_global(); l < L; l += wave_count()) {
-        float m = 0.f;
-        for (int e = l_off[l]; e < l_off[l + 1]; e++) m += dXc[(size_t)l_edges[e] * 128 + 64 + d];
-        dHl[(size_t)l * HID + d] = dXl[(size_t)l * 192 + d] + dXl[(size_t)(l ^ 1) * 192 + 128 + d] + 0.25f * m;
+    const int d = col4();
+    for (int l0 = 4 * wave_global(); l0 < L; l0 += 4 * wave_count()) {
+        const int l = l0 + row_slot();
+        if (l >= L) continue;
+        const float4 m = gather_sum(l_edges, l_off[l], l_off[l + 1], dXc, 128, 64 + d);
+        float4 h = ld4(dXl + (size_t)l * 192 + d);
+        add4(h, ld4(dXl + (size_t)(l ^ 1) * 192 + 128 + d));
+        add4(h, mul4(0.25f, m));
+        *reinterpret_cast<float4*>(dHl + (size_t)l * HID + d) = h;
     }
 }
 
@@ -480,6 +563,10 @@ constexpr int RED_BLOCKS = 4096;
 constexpr int RED_WAVES = 4 * RED_BLOCKS;
 constexpr int DW_BLOCKS = 1024;
 constexpr size_t PART_FLOATS = std::max<size_t>((size_t)DW_BLOCKS * 64 * 192, (size_t)RED_WAVES * O_WC);
+// Slot groups of the two-stage weight-gradient reduction, and their partials after
+// PART_FLOATS in the same buffer.
+constexpr int WIDE_GROUPS = 32;
+constexpr size_t PART2_FLOATS = (size_t)WIDE_GROUPS * 64 * 192;
 
 // One batch's activations, carved from the model's arena (see gpsat_gnn_model).
 struct Acts {
@@ -602,7 +689,13 @@ hipError_t acts_for(gpsat_gnn_model* m, gpsat_gnn_graph* g, Acts& a)
 template <bool ACC>
 void reduce(const float* part, int slots, int width, float* out, hipStream_t st)
 {
-    if (width >= 4096)
+    if (width >= 4096 && slots > 2 * WIDE_GROUPS && width % 4 == 0 && (size_t)width * WIDE_GROUPS <= PART2_FLOATS) {
+        float* part2 = const_cast<float*>(part) + PART_FLOATS;
+        hipLaunchKernelGGL(k_reduce_wide_split, dim3((width / 4 + 255) / 256, WIDE_GROUPS), dim3(256), 0, st, part,
+                           slots, width, part2);
+        hipLaunchKernelGGL(k_reduce_wide<ACC>, dim3((width + 255) / 256), dim3(256), 0, st, part2, WIDE_GROUPS, width,
+                           out);
+    } else if (width >= 4096)
         hipLaunchKernelGGL(k_reduce_wide<ACC>, dim3((width + 255) / 256), dim3(256), 0, st, part, slots, width, out);
     else
         hipLaunchKernelGGL(k_reduce_tree<ACC>, dim3(width), dim3(256), 0, st, part, slots, width, out);
@@ -628,11 +721,11 @@ hipError_t forward_pass(gpsat_gnn_model* m, gpsat_gnn_graph* g, const Acts& a)
     hipLaunchKernelGGL(k_embed, dim3((int)std::min<int64_t>((rows + 255) / 256, 65536)), dim3(256), 0, st, g->l_off,
                        g->c_off, g->L, g->C, m->th, a.Hl[0], a.Hc[0]);
     for (int t = 0; t < g->R; t++) {
-        hipLaunchKernelGGL(k_gather_clause, dim3(grid_rows(g->C)), dim3(256), 0, st, g->c_off, g->c_edges, g->C,
+        hipLaunchKernelGGL(k_gather_clause, dim3(grid_rows((g->C + 3) / 4)), dim3(256), 0, st, g->c_off, g->c_edges, g->C,
                            a.Hl[t], a.Hc[t], a.Xc[t]);
         hipLaunchKernelGGL((k_dense_fwd<128>), dim3((g->C + 63) / 64), dim3(256), 0, st, a.Xc[t], m->Wc,
                            m->th + O_BC, a.Hc[t + 1], g->C);
-        hipLaunchKernelGGL(k_gather_lit, dim3(grid_rows(g->L)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, a.Hl[t],
+        hipLaunchKernelGGL(k_gather_lit, dim3(grid_rows((g->L + 3) / 4)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, a.Hl[t],
                            a.Hc[t + 1], a.Xl[t]);
         hipLaunchKernelGGL((k_dense_fwd<192>), dim3((g->L + 63) / 64), dim3(256), 0, st, a.Xl[t], m->Wl,
                            m->th + O_BL, a.Hl[t + 1], g->L);
@@ -673,7 +766,7 @@ hipError_t backward_pass(gpsat_gnn_model* m, gpsat_gnn_graph* g, const Acts& a)
         reduce<true>(m->part, RED_WAVES, HID, m->grad + O_BC, st);
         dense_dw<128>(m, a.dP, a.Xc[t], g->C, m->grad + O_WC, st);
         hipLaunchKernelGGL((k_dense_bwd_dx<128>), dim3((g->C + 63) / 64), dim3(256), 0, st, a.dP, m->Wc, a.dXb, g->C);
-        hipLaunchKernelGGL(k_bwd_lit, dim3(grid_rows(g->L)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, a.dXa,
+        hipLaunchKernelGGL(k_bwd_lit, dim3(grid_rows((g->L + 3) / 4)), dim3(256), 0, st, g->l_off, g->l_edges, g->L, a.dXa,
                            a.dXb, a.dHl);
         self = a.dXb;
     }
@@ -712,7 +805,7 @@ int gpsat_gnn_model_create(const float* theta, gpsat_gnn_model** out)
     gpsat_gnn_model* m = new gpsat_gnn_model();
     hipError_t e = hipMalloc(&m->th, 4 * N_PARAMS);
     if (e == hipSuccess) e = hipMalloc(&m->grad, 4 * N_PARAMS);
-    if (e == hipSuccess) e = hipMalloc(&m->part, 4 * PART_FLOATS);
+    if (e == hipSuccess) e = hipMalloc(&m->part, 4 * (PART_FLOATS + PART2_FLOATS));
     if (e == hipSuccess) e = hipMalloc(&m->m, 4 * N_PARAMS);
     if (e == hipSuccess) e = hipMalloc(&m->v, 4 * N_PARAMS);
     if (e == hipSuccess) e = hipMalloc(&m->Wc, 2 * 64 * 128);
