@@ -301,7 +301,7 @@ def learned_leg(n: int, cube_ms: float):
     return out
 
 
-def general_leg(G, W, N: int, steps: int):
+def general_leg(G, W, N: int, steps: int, densities=(0.02, 0.05, 0.3), with_csr: bool = True):
     """The general eval/BCP kernel (any clause length, n and m) on the mixed k-SAT batch of
     the cube workload: 4,096 instances, k in {3,4,5} at ratios {4.26, 9.9, 21.1}, n in
     [100, 400], 64 rows each.  GB/s by the survey's byte accounting per instance."""
@@ -311,31 +311,54 @@ def general_leg(G, W, N: int, steps: int):
     ms_, ns_ = np.diff(b.inst_clause_off), b.n_vars
     byts = int(sum(survey_bytes_per_instance(int(n), int(m), int(L), A, True, True)
                    for n, m, L in zip(ns_, ms_, Ls)))
+    # compulsory bytes of the packed kernel: one 8-byte word pair per clause, rows in and out,
+    # counts, conflict flags and props, the instance's metadata and longest-clause byte
+    Ws = (2 * ns_.astype(np.int64) + 31) // 32
+    packed = int(np.sum(8 * ms_ + 8 * A * Ws + 20 * A + 33))
     out = {"workload": f"{N} mixed random k-SAT (k 3/4/5 at 4.26/9.9/21.1, n 100-400, m up to {int(ms_.max())}), "
-                       f"{A} rows each, fused eval + BCP through the general kernel",
-           "survey_bytes_per_launch": byts, "densities": {}}
+                       f"{A} rows each, fused eval + BCP through the packed general kernel (clauses of up to 6 "
+                       f"literals, n <= 496); the CSR general kernel timed beside it",
+           "survey_bytes_per_launch": byts, "packed_compulsory_bytes_per_launch": packed, "densities": {}}
+
+    def timed(gb, d_r, d_c, d_o, d_f, d_p):
+        run = lambda: gb.eval_bcp_dev(d_r, A, d_c, d_o, d_f, d_p)
+        run()
+        gb.sync()
+        e0, e1 = G.Event(), G.Event()
+        e0.record(gb)
+        for _ in range(steps):
+            run()
+        e1.record(gb)
+        gb.sync()
+        return e0.elapsed_ms(e1) / steps
+
     with G.GpuBatch(b) as gb:
-        for pd in (0.02, 0.05, 0.3):
+        out["kernel"] = gb.eval_kernel()
+        for pd in densities:
             rows = W.random_assignments(b, A, pd, seed=9)
             d_r = G.DevBuf.from_host(rows)
             d_c, d_o = G.DevBuf(N * A * 12), G.DevBuf(rows.nbytes)
             d_f, d_p = G.DevBuf(N * A * 4), G.DevBuf(N * A * 4)
-            run = lambda: gb.eval_bcp_dev(d_r, A, d_c, d_o, d_f, d_p)
-            run()
-            gb.sync()
-            e0, e1 = G.Event(), G.Event()
-            e0.record(gb)
-            for _ in range(steps):
-                run()
-            e1.record(gb)
-            gb.sync()
-            ms = e0.elapsed_ms(e1) / steps
-            cf = d_f.to_host(np.int32, N * A)
-            pr = d_p.to_host(np.int32, N * A)
+            ms = timed(gb, d_r, d_c, d_o, d_f, d_p)
+            got = (d_c.to_host(np.int32, N * A * 3), d_o.to_host(np.uint32, rows.size),
+                   d_f.to_host(np.int32, N * A), d_p.to_host(np.int32, N * A))
+            ms_csr = same = None
+            if with_csr:
+                with G.tuned(eval_pkk=0):
+                    ms_csr = timed(gb, d_r, d_c, d_o, d_f, d_p)
+                    same = all(np.array_equal(x, y) for x, y in zip(got, (
+                        d_c.to_host(np.int32, N * A * 3), d_o.to_host(np.uint32, rows.size),
+                        d_f.to_host(np.int32, N * A), d_p.to_host(np.int32, N * A))))
+            cf, pr = got[2], got[3]
             gbs = byts / (ms * 1e-3) / 1e9
             out["densities"][f"p{pd}"] = {"fused_ms": ms, "instances_per_s": N / (ms * 1e-3), "survey_gbs": gbs,
-                                          "frac_of_8tbs": gbs / HBM_PEAK_GBS, "conflict_frac": float(cf.mean()),
-                                          "implied_literals_per_s": float(pr.sum()) / (ms * 1e-3)}
+                                          "frac_of_8tbs": gbs / HBM_PEAK_GBS,
+                                          "packed_compulsory_gbs": packed / (ms * 1e-3) / 1e9,
+                                          "conflict_frac": float(cf.mean()),
+                                          "implied_literals_per_s": float(pr.sum()) / (ms * 1e-3),
+                                          "csr_kernel_ms": ms_csr,
+                                          "speedup_vs_csr": ms_csr / ms if ms_csr else None,
+                                          "outputs_equal_csr": same}
             for d in (d_r, d_c, d_o, d_f, d_p):
                 d.close()
     return out
@@ -362,7 +385,16 @@ def main():
     ap.add_argument("--mixed-instances", type=int, default=4096)
     ap.add_argument("--mixed-deadline-ms", type=float, default=20000.0)
     ap.add_argument("--learned-instances", type=int, default=256)
+    ap.add_argument("--general-only", type=str, default=None, metavar="P[,P...]",
+                    help="run only the general-kernel leg (packed kernel, no CSR comparison) at these trail "
+                         "densities and print its JSON (for profiling that kernel)")
     args = ap.parse_args()
+    if args.general_only:
+        import gpupsat_amd as G
+        from gpupsat_amd import workloads as W
+        dens = tuple(float(x) for x in args.general_only.split(","))
+        print(json.dumps(general_leg(G, W, args.mixed_instances, args.steps, dens, with_csr=False)))
+        return
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print("bench.py measures one GPU: the instance batch would shard across GPUs with no "
               "exchange between them, and multi-GPU runs are not measured here.", file=sys.stderr)

@@ -315,6 +315,13 @@ class GpuBatch:
     def n_inst(self) -> int:
         return self.batch.n_inst
 
+    def eval_kernel(self) -> str:
+        """The eval/BCP kernel launches on this batch run under the current tuning:
+        "fast3", "packed", "csr" or "csr_hbm" (gpsat_batch_eval_kernel)."""
+        k = ctypes.c_int32()
+        _check(lib().gpsat_batch_eval_kernel(self.h, ctypes.byref(k)))
+        return ("fast3", "packed", "csr", "csr_hbm")[k.value]
+
     def row_words(self, A: int) -> int:
         w = ctypes.c_int64(0)
         _check(lib().gpsat_batch_row_words(self.h, ctypes.c_int32(A), ctypes.byref(w)))

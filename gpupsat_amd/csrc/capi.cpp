@@ -23,6 +23,8 @@ Tuning& tuning()
         if (const char* e = getenv("GPSAT_TUNE_EVAL_STAGE")) x.eval_stage_arena = atoi(e);
         if (const char* e = getenv("GPSAT_TUNE_EVAL_FAST3")) x.eval_fast3 = atoi(e);
         if (const char* e = getenv("GPSAT_TUNE_EVAL_WPB")) x.eval_waves_per_block = atoi(e);
+        if (const char* e = getenv("GPSAT_TUNE_EVAL_PKK")) x.eval_pkk = atoi(e);
+        if (const char* e = getenv("GPSAT_TUNE_EVAL_PKK_WAVES")) x.eval_pkk_waves = atoi(e);
         return x;
     }();
     return t;
@@ -59,6 +61,8 @@ static int* tune_slot(const char* key)
     if (!strcmp(key, "eval_fast3")) return &tuning().eval_fast3;
     if (!strcmp(key, "eval_stage_arena")) return &tuning().eval_stage_arena;
     if (!strcmp(key, "eval_waves_per_block")) return &tuning().eval_waves_per_block;
+    if (!strcmp(key, "eval_pkk")) return &tuning().eval_pkk;
+    if (!strcmp(key, "eval_pkk_waves")) return &tuning().eval_pkk_waves;
     return nullptr;
 }
 
@@ -214,6 +218,24 @@ static void pack3_instance(const int32_t* L, int32_t m, uint32_t* out)
     }
 }
 
+// Packed words of one instance for the general path: clause c -> out[2c], out[2c + 1] with
+// slot k's literal code at bit 10 * (k % 3) of word k / 3; slots past the clause's length
+// hold the always-false slot code 32W + 1.
+static void packk_instance(const int32_t* lits, const int64_t* coff, int32_t m, int32_t W, uint32_t* out)
+{
+    const uint32_t fl = 32u * (uint32_t)W + 1u;
+    for (int32_t c = 0; c < m; c++) {
+        uint32_t w[2] = {0, 0};
+        const int64_t b = coff[c], len = coff[c + 1] - b;
+        for (int k = 0; k < kPkkMaxLen; k++) {
+            const uint32_t code = k < len ? (uint32_t)lits[b + k] : fl;
+            w[k / 3] |= code << (10 * (k % 3));
+        }
+        out[2 * c] = w[0];
+        out[2 * c + 1] = w[1];
+    }
+}
+
 int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int64_t* inst_clause_off,
                        const int32_t* n_vars, int32_t n_inst, gpsat_batch** out)
 {
@@ -239,7 +261,8 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
     d.h_coff.resize(M + n_inst);
     d.h_lits.assign(lits ? lits + l0 : nullptr, lits ? lits + l0 + L : nullptr);
     int64_t wbase = 0;
-    bool fast3 = true, every3 = true;
+    bool fast3 = true, every3 = true, packk = true;
+    std::vector<uint8_t> h_klen(n_inst, 0);
     for (int32_t i = 0; i < n_inst; i++) {
         const int64_t cb = inst_clause_off[i], ce = inst_clause_off[i + 1];
         if (ce < cb || n_vars[i] < 0) {
@@ -271,7 +294,14 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
             }
         }
         bool all3 = true;
-        for (int64_t c = cb; c < ce && all3; c++) all3 = clause_off[c + 1] - clause_off[c] == 3;
+        int64_t max_len = 0;
+        for (int64_t c = cb; c < ce; c++) {
+            const int64_t len = clause_off[c + 1] - clause_off[c];
+            all3 = all3 && len == 3;
+            max_len = std::max(max_len, len);
+        }
+        packk = packk && pkk_fits(mi.n, (int)std::min<int64_t>(max_len, INT32_MAX));
+        h_klen[i] = (uint8_t)std::min<int64_t>(max_len, 255);
         fast3 = fast3 && all3 && fast3_fits(mi.n, mi.m);
         every3 = every3 && all3;
         wbase += words_per_row(mi.n);
@@ -299,6 +329,18 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
         work(0);
         for (std::thread& th : pool) th.join();
     }
+    // the packed general kernel reads one 8-byte word pair per clause instead of the offsets
+    // and the literals (batches that the 3-literal kernel does not take)
+    std::vector<uint32_t> h_pkk;
+    const bool use_pkk = packk && !d.fast3 && n_inst > 0 && M > 0;
+    if (use_pkk) {
+        h_pkk.resize(2 * M);
+        for (int32_t i = 0; i < n_inst; i++) {
+            const int64_t cb = inst_clause_off[i];
+            packk_instance(lits, clause_off + cb, d.h_meta[i].m, words_per_row(d.h_meta[i].n),
+                           h_pkk.data() + 2 * (cb - c0));
+        }
+    }
     hipError_t e = hipGetDevice(&d.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
     if (e == hipSuccess && n_inst) e = hipMalloc(&d.meta, sizeof(InstMeta) * n_inst);
@@ -310,6 +352,18 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
         e = hipMemcpy(d.coff, d.h_coff.data(), sizeof(int32_t) * (M + n_inst), hipMemcpyHostToDevice);
     if (e == hipSuccess && d.fast3) e = hipMalloc(&d.pk3, sizeof(uint32_t) * M);
     if (e == hipSuccess && d.fast3) e = hipMemcpy(d.pk3, h_pk3.data(), sizeof(uint32_t) * M, hipMemcpyHostToDevice);
+    if (e == hipSuccess && use_pkk) e = hipMalloc(&d.pkk, sizeof(uint32_t) * 2 * M);
+    if (e == hipSuccess && use_pkk) e = hipMemcpy(d.pkk, h_pkk.data(), sizeof(uint32_t) * 2 * M, hipMemcpyHostToDevice);
+    if (e == hipSuccess && use_pkk) e = hipMalloc(&d.klen, n_inst);
+    if (e == hipSuccess && use_pkk) e = hipMemcpy(d.klen, h_klen.data(), n_inst, hipMemcpyHostToDevice);
+    if (e == hipSuccess && use_pkk) {
+        std::vector<int32_t> order(n_inst);
+        for (int32_t i = 0; i < n_inst; i++) order[i] = i;
+        const auto cost = [&](int32_t i) { return (int64_t)d.h_meta[i].m * std::max<int64_t>(3, h_klen[i]); };
+        std::stable_sort(order.begin(), order.end(), [&](int32_t a, int32_t c) { return cost(a) > cost(c); });
+        e = hipMalloc(&d.korder, sizeof(int32_t) * n_inst);
+        if (e == hipSuccess) e = hipMemcpy(d.korder, order.data(), sizeof(int32_t) * n_inst, hipMemcpyHostToDevice);
+    }
     if (e != hipSuccess) {
         set_error(std::string("batch upload: ") + hipGetErrorString(e));
         gpsat_batch_destroy(b);
@@ -328,9 +382,19 @@ int gpsat_batch_destroy(gpsat_batch* b)
     (void)hipFree(d.lits);
     (void)hipFree(d.coff);
     (void)hipFree(d.pk3);
+    (void)hipFree(d.pkk);
+    (void)hipFree(d.klen);
+    (void)hipFree(d.korder);
     (void)hipFree(d.gtab);
     if (d.stream) (void)hipStreamDestroy(d.stream);
     delete b;
+    return GPSAT_OK;
+}
+
+int gpsat_batch_eval_kernel(const gpsat_batch* b, int32_t* kind)
+{
+    ARG_CHECK(b && kind, "null argument");
+    *kind = (int32_t)eval_kernel(b->d);
     return GPSAT_OK;
 }
 

@@ -908,6 +908,335 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
     }
 }
 
+// ---------------------------------------------------------------------------------------
+// Packed general path: instances whose clauses hold at most 6 literals (any mix of lengths),
+// with n <= 496 (W <= 31 words per row) and any m.  Clause c is one 8-byte word pair of the
+// batch's pkk array: slot k's 10-bit literal code sits at bit 10 * (k % 3) of word k / 3.
+// Slots past the clause's length name the always-false slot 32W + 1, which is false in every
+// row and never implied, so a padded clause has the status and the unit literal of the
+// clause itself; lanes past m name the zero slot 32W (neither true nor false), which gives
+// no status and no unit.  The body runs for K = the instance's longest clause (at least 3),
+// so a 3-SAT instance reads three slots per clause and a 5-SAT one five.
+//
+// One workgroup of NW wavefronts per instance: the waves share the instance's mask table in
+// LDS (the 3-literal kernel's slot l = {T(l), T(l ^ 1)}, 16 bytes, partner slot at address
+// bit 4) and take its clause groups in turn (wave v: groups v, v + NW, ...), so a long
+// instance is swept NW times faster than by one wave.  Implications go straight into the
+// shared table; each BCP pass ends at one barrier, where the waves' "implied something"
+// rows and conflict rows are combined in LDS.  The table is built straight from the input
+// rows in HBM and the closure rows go straight back.
+//
+// LDS per workgroup (words): the table, 128W + 8; then the conflict rows (2), three
+// "implied something" words of 2 (pass p uses slot p % 3: the slot the next pass ORs into
+// is cleared before this pass's barrier, after every wave has read it two barriers earlier),
+// and 192 count totals.
+constexpr int kPkAhead = 4;
+
+__host__ __device__ constexpr int pk_lds_words(int W) { return 128 * W + 8 + 8 + 192; }
+
+template <int K>
+struct PkGroup {
+    uint32_t ad[K];     // slot byte addresses
+    u32x4 q[K];         // slot contents {T lo, T hi, F lo, F hi}
+};
+
+template <int K>
+__device__ __forceinline__ void pk_fetch(uint2 w, uint32_t tabb, PkGroup<K>& gr)
+{
+#pragma unroll
+    for (int k = 0; k < K; k++) {
+        const uint32_t word = k < 3 ? w.x : w.y;
+        gr.ad[k] = tabb + 16u * __builtin_amdgcn_ubfe(word, 10 * (k % 3), 10);
+    }
+#pragma unroll
+    for (int k = 0; k < K; k++) gr.q[k] = lds_read16(gr.ad[k]);
+}
+
+// Per 32-bit half h: T = some literal true, N0 = every literal false, N1 = exactly one
+// literal not false.
+template <int K>
+__device__ __forceinline__ void pk_status(const PkGroup<K>& gr, uint32_t (&T)[2], uint32_t (&N0)[2],
+                                          uint32_t (&N1)[2])
+{
+    if (K == 3) {
+        T[0] = gr.q[0].x | gr.q[1].x | gr.q[2].x;
+        T[1] = gr.q[0].y | gr.q[1].y | gr.q[2].y;
+        N0[0] = and3(gr.q[0].z, gr.q[1].z, gr.q[2].z);
+        N0[1] = and3(gr.q[0].w, gr.q[1].w, gr.q[2].w);
+        N1[0] = two3(gr.q[0].z, gr.q[1].z, gr.q[2].z);
+        N1[1] = two3(gr.q[0].w, gr.q[1].w, gr.q[2].w);
+        return;
+    }
+    T[0] = gr.q[0].x;
+    T[1] = gr.q[0].y;
+    N0[0] = gr.q[0].z;
+    N0[1] = gr.q[0].w;
+    N1[0] = ~gr.q[0].z;
+    N1[1] = ~gr.q[0].w;
+#pragma unroll
+    for (int k = 1; k < K; k++) {
+        T[0] |= gr.q[k].x;
+        T[1] |= gr.q[k].y;
+        // exactly one non-false so far: it was before and this literal is false, or none was
+        // and this one is not
+        N1[0] = bfi(gr.q[k].z, N1[0], N0[0]);
+        N1[1] = bfi(gr.q[k].w, N1[1], N0[1]);
+        N0[0] &= gr.q[k].z;
+        N0[1] &= gr.q[k].w;
+    }
+}
+
+// One sweep over this wave's clause groups g0, g0 + step, ...: a group's clause words arrive
+// kPkAhead of the wave's groups ahead of their use, and its slot reads are issued before the
+// previous group is processed (they may then miss that group's implications until the next
+// pass; a BCP pass ends the loop only when no wave wrote anything, so the fixpoint is
+// unchanged).  process(group) returns true to stop the sweep.
+template <int K, class F>
+__device__ __forceinline__ void pk_sweep(const uint2* __restrict__ ipk, int m, int G, int g0, int step, int lane,
+                                         uint32_t tabb, uint2 zw, F&& process)
+{
+    const int cnt = G > g0 ? (G - g0 + step - 1) / step : 0;   // this wave's groups
+    const auto grp = [&](int t) { return g0 + t * step; };
+    const auto word = [&](uint2 w, int t) { return lane < m - grp(t) * kWave ? w : zw; };
+    // lanes past m re-read clause m - 1 (in bounds) and are given the zero slots
+    const auto load = [&](int t) -> uint2 { return ipk[grp(t) * kWave + min(lane, m - 1 - grp(t) * kWave)]; };
+    uint2 pw[kPkAhead];
+#pragma unroll
+    for (int u = 0; u < kPkAhead; u++) pw[u] = u < cnt ? load(u) : zw;
+    PkGroup<K> buf[2];
+    if (cnt > 0) pk_fetch<K>(word(pw[0], 0), tabb, buf[0]);
+    bool stop = false;
+    for (int t0 = 0; t0 < cnt && !stop; t0 += kPkAhead) {
+#pragma unroll
+        for (int u = 0; u < kPkAhead; u++) {
+            const int t = t0 + u;
+            if (t < cnt && !stop) {
+                if (t + 1 < cnt) pk_fetch<K>(word(pw[(u + 1) % kPkAhead], t + 1), tabb, buf[(u + 1) & 1]);
+                if (t + kPkAhead < cnt) pw[u] = load(t + kPkAhead);
+                stop = process(buf[u & 1]);
+            }
+        }
+    }
+}
+
+__device__ __forceinline__ void lds_or64p(uint64_t* p, uint64_t v)
+{
+    __hip_atomic_fetch_or((unsigned long long*)p, (unsigned long long)v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+__device__ __forceinline__ uint64_t lds_ld64(const uint64_t* p)
+{
+    return __hip_atomic_load((const unsigned long long*)p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+template <int K, bool DO_EVAL, bool DO_BCP>
+__device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int wv, int NW, int lane, uint32_t* lds,
+                                               const uint2* __restrict__ pkk, const uint32_t* __restrict__ rows, int A,
+                                               int32_t* __restrict__ counts, uint32_t* __restrict__ out_rows,
+                                               int32_t* __restrict__ conflict, int32_t* __restrict__ props,
+                                               int32_t* __restrict__ passes_out)
+{
+    const int W = __builtin_amdgcn_readfirstlane(words_per_row(mi.n));
+    const int m = __builtin_amdgcn_readfirstlane(mi.m);
+    const int G = (m + kWave - 1) / kWave;
+    const int64_t row0 = (int64_t)A * uni64(mi.word_base);
+    const uint2* ipk = pkk + uni64(mi.coff_base - inst);
+    const uint32_t* src = rows + row0;
+    uint32_t* tab = lds;
+    const uint32_t tabb = lds_addr(tab);
+    const u32x4* tab4 = reinterpret_cast<const u32x4*>(tab);
+    uint64_t* sh_frozen = reinterpret_cast<uint64_t*>(lds + 128 * W + 8);
+    uint64_t* sh_chg = sh_frozen + 1;                        // [3]
+    uint32_t* sh_tot = lds + 128 * W + 16;                   // [64][3]
+    const uint32_t zslot = 32u * (uint32_t)W;
+    const uint32_t zc = zslot | (zslot << 10) | (zslot << 20);
+    const uint2 zw = make_uint2(zc, zc);
+    const Transposer tp(lane);
+    const int half = lane >> 5, i = lane & 31;
+    const int tid = wv * kWave + lane, nthreads = NW * kWave;
+
+    // the mask table: lane a reads word j of row a, the transpose hands lane (half, i) the
+    // masks of literal 32j + i over rows 32*half .. 32*half + 31 (wave v: words v, v + NW, ...)
+    for (int j0 = wv; j0 < W; j0 += 8 * NW) {
+        uint32_t r[8];
+#pragma unroll
+        for (int u = 0; u < 8; u++) r[u] = (j0 + u * NW < W && lane < A) ? src[lane * W + j0 + u * NW] : 0u;
+#pragma unroll
+        for (int u = 0; u < 8; u++) {
+            const int j = j0 + u * NW;
+            if (j < W) {
+                const uint32_t x = tp(r[u]);
+                const int l = 32 * j + i;
+                tab[4 * l + half] = x;
+                tab[4 * (l ^ 1) + 2 + half] = x;
+            }
+        }
+    }
+    if (wv == 0 && lane < 8) tab[4 * zslot + lane] = lane < 6 ? 0u : ~0u;   // zero slot, then always-false slot
+    for (int k = tid; k < 8 + 192; k += nthreads) lds[128 * W + 8 + k] = 0u;
+    __syncthreads();
+
+    uint64_t unsat_any = 0;
+    if (DO_EVAL) {
+        uint32_t acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
+        pk_sweep<K>(ipk, m, G, wv, NW, lane, tabb, zw, [&](const PkGroup<K>& gr) {
+            uint32_t T[2], N0[2], N1[2];
+            pk_status<K>(gr, T, N0, N1);
+            // SAT|UNIT and UNSAT|UNIT per clause
+            const uint64_t b0 = u64(T[0] | N1[0], T[1] | N1[1]);
+            const uint64_t b1 = u64((N0[0] | N1[0]) & ~T[0], (N0[1] | N1[1]) & ~T[1]);
+            count_group(b0, b1, tp, acc);
+            return false;
+        });
+        uint32_t tot[3];
+        finish_counts(acc, lane, tot);
+        if (lane < A) {
+#pragma unroll
+            for (int s = 0; s < 3; s++) atomicAdd(&sh_tot[3 * lane + s], tot[s]);
+        }
+        __syncthreads();
+        const bool my_unsat = lane < A && sh_tot[3 * lane + 1] != 0u;
+        if (wv == 0 && lane < A) {
+            int32_t* o = counts + ((int64_t)inst * A + lane) * 3;
+            o[0] = (int32_t)sh_tot[3 * lane];
+            o[1] = (int32_t)sh_tot[3 * lane + 1];
+            o[2] = (int32_t)sh_tot[3 * lane + 2];
+        }
+        unsat_any = ballot(my_unsat);
+    }
+
+    if (DO_BCP) {
+        const uint64_t amask = A >= 64 ? ~0ull : ((1ull << A) - 1ull);
+        // contradictory inputs, and inputs that already falsify a clause, are conflicts
+        {
+            uint64_t cm = unsat_any;
+            for (int v = tid; v < 16 * W; v += nthreads) {
+                const u32x4 q = tab4[2 * v];
+                cm |= u64(q.x, q.y) & u64(q.z, q.w);
+            }
+            cm = wave_or64(cm);
+            if (lane == 0 && cm) lds_or64p(sh_frozen, cm);
+        }
+        __syncthreads();
+        // Every wave leaves the pass loop on the same value: the pass's "moved" slot, which no
+        // wave writes between the barrier that ends the pass and the next pass's barrier.  The
+        // conflict rows only grow, and any snapshot of them is a valid one.
+        uint64_t frozen = 0;
+        int pass = 0;
+        for (;;) {
+            pass++;
+            frozen |= lds_ld64(sh_frozen);
+            uint64_t chg = 0;       // rows in which this lane's clauses implied a literal
+            if ((frozen & amask) != amask) pk_sweep<K>(ipk, m, G, wv, NW, lane, tabb, zw, [&](const PkGroup<K>& gr) {
+                uint32_t T[2], N0[2], N1[2];
+                pk_status<K>(gr, T, N0, N1);
+                const uint64_t open = ~(u64(T[0], T[1]) | frozen);
+                const uint64_t newconf = u64(N0[0], N0[1]) & open;
+                const uint64_t U = u64(N1[0], N1[1]) & open;
+                if (U) {
+                    // the unit literal is the one slot that is not false
+#pragma unroll
+                    for (int k = 0; k < K; k++) {
+                        const uint64_t Uk = U & ~u64(gr.q[k].z, gr.q[k].w);
+                        if (Uk) {
+                            lds_or64(gr.ad[k], Uk);
+                            lds_or64((gr.ad[k] ^ 16u) + 8u, Uk);
+                        }
+                    }
+                }
+                chg |= U;
+                if (ballot(newconf != 0)) {
+                    frozen |= wave_or64(newconf);
+                    return (frozen & amask) == amask;
+                }
+                return false;
+            });
+            // rows this wave found in conflict, rows it still moved; the next pass's slot is
+            // cleared before the barrier (every wave read it two barriers ago)
+            const uint64_t moved = wave_or64(chg) & ~frozen;
+            if (lane == 0) {
+                if (frozen) lds_or64p(sh_frozen, frozen);
+                if (moved) lds_or64p(&sh_chg[pass % 3], moved);
+                if (wv == 0) __hip_atomic_store((unsigned long long*)&sh_chg[(pass + 1) % 3], 0ull, __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_WORKGROUP);
+            }
+            __syncthreads();
+            if ((lds_ld64(&sh_chg[pass % 3]) & amask) == 0ull) break;
+        }
+        // complementary implications are conflicts too
+        {
+            uint64_t cm = 0;
+            for (int v = tid; v < 16 * W; v += nthreads) {
+                const u32x4 q = tab4[2 * v];
+                cm |= u64(q.x, q.y) & u64(q.z, q.w);
+            }
+            cm = wave_or64(cm) | frozen;
+            if (lane == 0 && cm) lds_or64p(sh_frozen, cm);
+        }
+        __syncthreads();
+        if (wv != 0) return;
+        frozen = lds_ld64(sh_frozen);
+        const bool my_conflict = (frozen >> lane) & 1ull;
+        const bool all_conflict = (frozen & amask) == amask;
+        uint32_t* orow = out_rows + row0;
+        int cnt = 0, in_pop = 0;
+        for (int j0 = 0; j0 < W; j0 += 8) {
+            uint32_t x[8];
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                x[u] = 0u;
+                if (j0 + u < W && !all_conflict) x[u] = tp(tab[4 * (32 * (j0 + u) + i) + half]);   // lane a: row a
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                if (j0 + u < W && lane < A) {
+                    const uint32_t in = src[lane * W + j0 + u];
+                    const uint32_t o = my_conflict ? in : x[u];
+                    in_pop += __builtin_popcount(in);
+                    cnt += __builtin_popcount(o);
+                    orow[lane * W + j0 + u] = o;
+                }
+            }
+        }
+        // the closure contains the input, so the implied literals are the difference of the counts
+        if (lane < A) {
+            conflict[(int64_t)inst * A + lane] = my_conflict ? 1 : 0;
+            props[(int64_t)inst * A + lane] = cnt - in_pop;
+        }
+        if (passes_out && lane == 0) passes_out[inst] = pass;
+    }
+}
+
+// Workgroup b takes instance order[b]: the batch's instances by decreasing m * K, so the
+// longest sweeps start first and the short ones fill the end of the launch.
+template <bool DO_EVAL, bool DO_BCP>
+__global__ __launch_bounds__(256) void k_eval_bcpk(const InstMeta* __restrict__ meta, const uint2* __restrict__ pkk,
+                                                    const uint8_t* __restrict__ klen, const int32_t* __restrict__ order,
+                                                    const uint32_t* __restrict__ rows, int A,
+                                                    int32_t* __restrict__ counts, uint32_t* __restrict__ out_rows,
+                                                    int32_t* __restrict__ conflict, int32_t* __restrict__ props,
+                                                    int32_t* __restrict__ passes_out)
+{
+    extern __shared__ __align__(32) uint32_t lds_all[];
+    const int NW = blockDim.x / kWave;
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const int lane = threadIdx.x % kWave;
+    const int inst = __builtin_amdgcn_readfirstlane(order[blockIdx.x]);
+    const InstMeta mi = meta[inst];
+    const int K = __builtin_amdgcn_readfirstlane((int)klen[inst]);
+    if (K <= 3)
+        eval_bcpk_body<3, DO_EVAL, DO_BCP>(mi, inst, wv, NW, lane, lds_all, pkk, rows, A, counts, out_rows, conflict,
+                                           props, passes_out);
+    else if (K == 4)
+        eval_bcpk_body<4, DO_EVAL, DO_BCP>(mi, inst, wv, NW, lane, lds_all, pkk, rows, A, counts, out_rows, conflict,
+                                           props, passes_out);
+    else if (K == 5)
+        eval_bcpk_body<5, DO_EVAL, DO_BCP>(mi, inst, wv, NW, lane, lds_all, pkk, rows, A, counts, out_rows, conflict,
+                                           props, passes_out);
+    else
+        eval_bcpk_body<6, DO_EVAL, DO_BCP>(mi, inst, wv, NW, lane, lds_all, pkk, rows, A, counts, out_rows, conflict,
+                                           props, passes_out);
+}
+
 // LDS words of one wavefront's region in the general kernel (literal masks, row staging,
 // conflict words; plus the clause arena when it is staged).
 static int general_lds_words(int max_n) { return 128 * words_per_row(max_n) + 8; }
@@ -953,11 +1282,19 @@ static int waves_per_block(int lds_words)
     return w;
 }
 
+EvalKernel eval_kernel(const DeviceBatch& b)
+{
+    if (b.fast3 && tuning().eval_fast3) return EK_FAST3;
+    if (b.pkk && tuning().eval_pkk) return EK_PACKED;
+    const bool stage = tuning().eval_stage_arena && (size_t)stage_lds_words(b) * 4 <= 40 * 1024;
+    if (!stage && (size_t)general_lds_words(b.max_n) * 4 > 64 * 1024) return EK_CSR_HBM;
+    return EK_CSR;
+}
+
 size_t eval_bcp_gtab_words(const DeviceBatch& b)
 {
-    if (b.n_inst == 0 || (b.fast3 && tuning().eval_fast3)) return 0;
+    if (b.n_inst == 0 || eval_kernel(b) != EK_CSR_HBM) return 0;
     const int lds_words = general_lds_words(b.max_n);
-    if ((size_t)lds_words * 4 <= 64 * 1024) return 0;
     return (size_t)((lds_words + 63) & ~63) * b.n_inst;
 }
 
@@ -966,7 +1303,8 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
                            hipStream_t st, uint32_t* gtab_ext)
 {
     if (b.n_inst == 0) return hipSuccess;
-    if (b.fast3 && tuning().eval_fast3) {
+    const EvalKernel ek = eval_kernel(b);
+    if (ek == EK_FAST3) {
         const int lds_words = 128 * words_per_row(b.max_n) + 8;   // 32-byte multiple
         const int wpb = waves_per_block(lds_words);
         dim3 grid((b.n_inst + wpb - 1) / wpb), block(kWave * wpb);
@@ -982,6 +1320,23 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
                                out_rows, conflict, props, passes, b.n_inst, lds_words);
         return hipGetLastError();
     }
+    if (ek == EK_PACKED) {
+        // one workgroup of eval_pkk_waves wavefronts per instance (the largest instance's table)
+        const int nw = std::max(1, std::min(4, tuning().eval_pkk_waves));
+        dim3 grid(b.n_inst), block(kWave * nw);
+        const size_t lds = (size_t)pk_lds_words(words_per_row(b.max_n)) * 4;
+        const uint2* pk = reinterpret_cast<const uint2*>(b.pkk);
+        if (do_eval && do_bcp)
+            hipLaunchKernelGGL((k_eval_bcpk<true, true>), grid, block, lds, st, b.meta, pk, b.klen, b.korder, rows, A,
+                               counts, out_rows, conflict, props, passes);
+        else if (do_eval)
+            hipLaunchKernelGGL((k_eval_bcpk<true, false>), grid, block, lds, st, b.meta, pk, b.klen, b.korder, rows, A,
+                               counts, out_rows, conflict, props, passes);
+        else
+            hipLaunchKernelGGL((k_eval_bcpk<false, true>), grid, block, lds, st, b.meta, pk, b.klen, b.korder, rows, A,
+                               counts, out_rows, conflict, props, passes);
+        return hipGetLastError();
+    }
     // stage the arena in LDS when the largest instance keeps the wavefront under 40 KB
     const int staged = stage_lds_words(b);
     const bool stage = tuning().eval_stage_arena && (size_t)staged * 4 <= 40 * 1024;
@@ -989,7 +1344,7 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
     // mask tables beyond a workgroup's 64 KB of LDS go to an HBM slice per instance, held
     // by the batch from the first such launch on
     uint32_t* gtab = nullptr;
-    if (!stage && (size_t)lds_words * 4 > 64 * 1024) {
+    if (ek == EK_CSR_HBM) {
         lds_words = (lds_words + 63) & ~63;
         const size_t need = (size_t)lds_words * b.n_inst;
         if (gtab_ext) {

@@ -26,6 +26,12 @@ struct DeviceBatch {
     // device [n_clauses] when fast3: clause c's three literal codes in bits 0-9, 10-19 and
     // 20-29 (codes are below 2n <= 512); instance i's clauses start at meta.coff_base - i
     uint32_t* pk3 = nullptr;
+    // device [2 * n_clauses] when every instance fits the packed general path (and the batch
+    // is not fast3): clause c's word pair, six 10-bit slots (see pkk_fits); klen [n_inst]
+    // holds each instance's longest clause
+    uint32_t* pkk = nullptr;
+    uint8_t* klen = nullptr;
+    int32_t* korder = nullptr;    // device [n_inst]: instances by decreasing m * max(3, klen)
     // HBM slices for the eval/BCP mask tables of instances too large for LDS (allocated by
     // the first launch that needs them, freed with the batch)
     mutable uint32_t* gtab = nullptr;
@@ -45,6 +51,8 @@ struct Tuning {
     int eval_stage_arena = 0;     // copy each instance's clause arena to LDS before evaluating
     int eval_fast3 = 1;           // use the register-resident kernel when the batch allows it
     int eval_waves_per_block = 4; // independent wavefronts (instances) per workgroup: 1, 2 or 4
+    int eval_pkk = 1;             // use the packed general kernel when the batch allows it
+    int eval_pkk_waves = 4;       // wavefronts sharing one instance in the packed kernel: 1-4
 };
 Tuning& tuning();
 
@@ -54,12 +62,23 @@ constexpr int kFast3MaxWords = 16;
 constexpr int kFast3MaxGroups = 16;
 inline bool fast3_fits(int n, int m) { return words_per_row(n) <= kFast3MaxWords && m <= 64 * kFast3MaxGroups; }
 
+// Limits of the packed general kernel: clauses of at most 6 literals, W <= 31 words per row
+// (n <= 496, so every literal code and the two special slot codes 32W and 32W + 1 fit 10
+// bits), any m.
+constexpr int kPkkMaxWords = 31;
+constexpr int kPkkMaxLen = 6;
+inline bool pkk_fits(int n, int max_len) { return words_per_row(n) <= kPkkMaxWords && max_len <= kPkkMaxLen; }
+
 size_t eval_bcp_lds_bytes(int max_n);
 // gtab_ext: caller-provided HBM slices for the mask tables of instances too large for LDS
 // (eval_bcp_gtab_words of them); null: the batch's own, allocated on first use.
 hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, int32_t* counts, uint32_t* out_rows,
                            int32_t* conflict, int32_t* props, int32_t* passes, bool do_eval, bool do_bcp,
                            hipStream_t st, uint32_t* gtab_ext = nullptr);
+// The kernel an eval/BCP launch on this batch runs under the current tuning (the kinds of
+// gpsat_batch_eval_kernel).
+enum EvalKernel { EK_FAST3 = 0, EK_PACKED = 1, EK_CSR = 2, EK_CSR_HBM = 3 };
+EvalKernel eval_kernel(const DeviceBatch& b);
 // Words of HBM mask-table slices a general eval/BCP launch on this batch needs (0: none).
 size_t eval_bcp_gtab_words(const DeviceBatch& b);
 

@@ -73,8 +73,12 @@ int gpsat_copy_dev(gpsat_batch *b, void *dst_dev, const void *src_dev, int64_t b
  *   "eval_stage_arena"  1: the general kernel copies each clause arena to LDS first (default 0)
  *   "eval_waves_per_block"  instances (wavefronts) per workgroup of the eval/BCP kernels:
  *                       1, 2 or 4 (default 4)
- * The GPSAT_TUNE_EVAL_FAST3 / GPSAT_TUNE_EVAL_STAGE / GPSAT_TUNE_EVAL_WPB environment
- * variables set the initial values.  Returns GPSAT_ERR_ARG for an unknown key. */
+ *   "eval_pkk"          1: evaluate batches of clauses with at most 6 literals and n <= 496
+ *                       (that the register-resident kernel does not take) with the packed
+ *                       general kernel (default); 0: the CSR general kernel
+ * The GPSAT_TUNE_EVAL_FAST3 / GPSAT_TUNE_EVAL_STAGE / GPSAT_TUNE_EVAL_WPB /
+ * GPSAT_TUNE_EVAL_PKK environment variables set the initial values.  Returns GPSAT_ERR_ARG
+ * for an unknown key. */
 int gpsat_tune_set(const char *key, int32_t value);
 int gpsat_tune_get(const char *key, int32_t *value);
 
@@ -84,13 +88,20 @@ int gpsat_tune_get(const char *key, int32_t *value);
  * lits[clause_off[c] .. clause_off[c+1]) is clause c, instance i owns clauses
  * inst_clause_off[i] .. inst_clause_off[i+1], n_vars[i] variables.  Device memory: 4 B per
  * literal, 4 B per clause offset and 32 B per instance; when every clause has 3 literals,
- * n <= 256 and m <= 1024, also 4 B per clause of packed literal triples. */
+ * n <= 256 and m <= 1024, also 4 B per clause of packed literal triples; otherwise, when
+ * every clause has at most 6 literals and n <= 496, 8 B per clause of packed literal codes
+ * and 1 B per instance. */
 int gpsat_batch_create(const int32_t *lits, const int64_t *clause_off, const int64_t *inst_clause_off,
                        const int32_t *n_vars, int32_t n_inst, gpsat_batch **out);
 int gpsat_batch_destroy(gpsat_batch *b);
 int gpsat_batch_info(const gpsat_batch *b, int32_t *n_inst, int64_t *n_clauses, int64_t *n_lits, int32_t *max_vars);
 int gpsat_batch_row_words(const gpsat_batch *b, int32_t A, int64_t *total_words);
 int gpsat_batch_sync(gpsat_batch *b);
+/* Which kernel an eval/BCP launch on this batch runs under the current tuning: 0 the
+ * register-resident 3-literal kernel, 1 the packed general kernel, 2 the CSR general kernel
+ * with its mask table in LDS, 3 the CSR general kernel with mask tables in HBM slices
+ * (n too large for LDS).  No reference counterpart (diagnostics). */
+int gpsat_batch_eval_kernel(const gpsat_batch *b, int32_t *kind);
 
 /* ---- a4: clause evaluation, VariablesStateHandler::clause_status
  * (src/SATSolver/VariablesStateHandler.cu:180-206) for every clause under every

@@ -255,3 +255,69 @@ def test_large_instances_hbm_mask_table(orc, A):
     rows = W.random_assignments(b, A, 0.02, seed=35)
     counts, conflict, props = check_batch(orc, b, rows, A)
     assert props.sum() > 0
+
+
+def mixed_length_formula(n, m, rng):
+    """m clauses over n vars with lengths 1-6 (1% units, 10% binary, the rest 3-6)."""
+    cls = []
+    for _ in range(m):
+        k = int(rng.choice([1, 2, 3, 4, 5, 6], p=[0.01, 0.1, 0.22, 0.23, 0.22, 0.22]))
+        vs = rng.choice(n, size=k, replace=False)
+        cls.append([2 * int(v) + int(rng.integers(2)) for v in vs])
+    return Formula.from_clauses(n, cls)
+
+
+def packed_batch():
+    rng = np.random.default_rng(61)
+    mixed, _ = W.mixed_ksat_batch(8, seed=62)
+    fs = [mixed.instance(i) for i in range(mixed.n_inst)]
+    fs += [
+        W.random_3sat_batch(1, 300, 1278, seed=63).instance(0),   # 3-SAT beyond the 3-literal kernel
+        Batch.uniform(W.random_ksat_clauses(np.random.default_rng(64), 1, 70, 2900, 6), 70).instance(0),  # 6-SAT
+        mixed_length_formula(496, 1500, rng),                        # W = 31, the largest packed n
+        mixed_length_formula(37, 90, rng),
+        Formula.from_clauses(3, []),                                 # no clauses
+        Formula.from_clauses(3, [[], [1, 3]]),                       # empty clause
+        Formula.from_clauses(4, [[1], [0, 3], [2, 5], [4, 7]]),      # unit chain
+        Formula.from_clauses(3, [[1, 1, 3], [0, 1, 5], [2, 2]]),     # duplicates, tautology
+        Formula.from_clauses(1, [[1], [0]]),
+        Formula.from_clauses(6, [[0, 2, 4, 6, 8, 10], [1, 3, 5, 7, 9, 11], [11, 9]]),
+    ]
+    return Batch.from_formulas(fs)
+
+
+def packed_rows(b, A, p, seed):
+    rows = W.random_assignments(b, A, p, seed=seed)
+    # row 3 of every instance (A > 3): a contradictory input
+    if A > 3:
+        base = 0
+        for i in range(b.n_inst):
+            w = (2 * int(b.n_vars[i]) + 31) // 32
+            if w:
+                rows[base + 3 * w] |= 0x3
+            base += A * w
+    return rows
+
+
+@pytest.mark.parametrize("A,p,nw", [(64, 0.02, 4), (64, 0.05, 4), (64, 0.15, 4), (64, 0.3, 4), (1, 0.05, 4),
+                                    (37, 0.1, 4), (64, 0.05, 1), (64, 0.1, 3), (64, 0.3, 2)])
+def test_packed_general_kernel(orc, A, p, nw):
+    """The packed general kernel (clauses of 1-6 literals in any mix, n up to 496, m up to
+    2,900, empty and unit clauses, duplicate literals, contradictory inputs; nw wavefronts
+    sharing each instance) gives the oracle's counts, closure rows, conflicts and props bit
+    for bit, and the same outputs as the CSR general kernel."""
+    print(f"[eval/bcp] packed general kernel A={A} p={p} waves={nw}", flush=True)
+    b = packed_batch()
+    rows = packed_rows(b, A, p, seed=int(1000 * p) + A)
+    with G.GpuBatch(b) as gb:
+        assert gb.eval_kernel() == "packed"
+        with G.tuned(eval_pkk=0):
+            assert gb.eval_kernel() == "csr"
+            c_counts, c_out, c_conflict, c_props = fused(gb, rows, A)
+    with G.tuned(eval_pkk_waves=nw):
+        counts, conflict, props = check_batch(orc, b, rows, A)
+    np.testing.assert_array_equal(counts, c_counts)
+    np.testing.assert_array_equal(conflict, c_conflict)
+    np.testing.assert_array_equal(props, c_props)
+    if p <= 0.05 and A == 64:
+        assert props.sum() > 0 and conflict.mean() < 0.9

@@ -346,3 +346,43 @@ def test_large_formula_beyond_lds(orc):
         check_models(orc, gb, res, b)
     assert res["info"]["clause_mem"] == 4
     assert res["status"].tolist() == [G.SAT, G.SAT, G.UNSAT]
+
+
+def restart_points(first, factor, upto):
+    """Conflict counts T_1, T_2, ... at which the geometric schedule (first, first*factor,
+    ... conflicts apart, each step truncated to an integer in FP32 as the solver computes
+    it) allows restart 1, 2, ...; the schedule of Configs.cuh:104-105 is first=100, 1.3."""
+    pts, t, thr = [], 0, first
+    while True:
+        t += thr
+        if t > upto:
+            return pts
+        pts.append(t)
+        thr = int(np.float32(thr) * np.float32(factor))
+
+
+@pytest.mark.parametrize("first,factor", [(100, 1.3), (40, 2.0)])
+def test_restart_schedule(orc, first, factor):
+    """k_solve's restart count follows the geometric schedule: restart k needs the sum of the
+    first k intervals in conflicts, so R <= #{T_k <= C}; a SAT job ends right after a
+    conflict-free propagation, where a due restart would already have run, so R falls short
+    of that count only when conflicts ran on past a due restart point."""
+    print(f"[solve] restart schedule first={first} factor={factor}", flush=True)
+    b = W.random_3sat_batch(150, 110, 469, seed=77)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(restart_first=first, restart_factor=factor, learnt_cap=4096, max_ms=0.0))
+        check_models(orc, gb, res, b)
+    js = res["job_stats"]
+    st = res["status"]
+    assert (st != G.UNDEF).all()
+    exact = sat = 0
+    for i in range(b.n_inst):
+        c, r = int(js[i, 1]), int(js[i, 5])
+        bound = len(restart_points(first, factor, c))
+        assert r <= bound, (i, c, r, bound)
+        if st[i] == G.SAT:
+            sat += 1
+            assert r >= bound - 1, (i, c, r, bound)
+            exact += r == bound
+    assert js[:, 5].sum() >= 100
+    assert sat >= 20 and exact >= 0.8 * sat, (exact, sat)
