@@ -385,10 +385,44 @@ def main():
     ap.add_argument("--mixed-instances", type=int, default=4096)
     ap.add_argument("--mixed-deadline-ms", type=float, default=20000.0)
     ap.add_argument("--learned-instances", type=int, default=256)
+    ap.add_argument("--compare-packed", action="store_true",
+                    help="time the packed general kernel beside the 3-literal kernel on the headline batch over the "
+                         "sweep's trail densities (sets GPSAT_PKK_FOR_FAST3) and print that comparison only")
     ap.add_argument("--general-only", type=str, default=None, metavar="P[,P...]",
                     help="run only the general-kernel leg (packed kernel, no CSR comparison) at these trail "
                          "densities and print its JSON (for profiling that kernel)")
     args = ap.parse_args()
+    if args.compare_packed:
+        os.environ["GPSAT_PKK_FOR_FAST3"] = "1"
+        import gpupsat_amd as G
+        from gpupsat_amd import workloads as W
+        b = W.random_3sat_batch(args.instances, 200, 852, seed=0)
+        A, res = 64, {}
+        with G.GpuBatch(b) as gb:
+            d_c, d_f, d_p = G.DevBuf(b.n_inst * A * 12), G.DevBuf(b.n_inst * A * 4), G.DevBuf(b.n_inst * A * 4)
+            for pd in (0.02, 0.05, 0.1, 0.3):
+                r = W.random_assignments(b, A, pd, seed=2)
+                d_r, d_o = G.DevBuf.from_host(r), G.DevBuf(r.nbytes)
+                row = {}
+                for name, kw in (("fast3", {}), ("packed_nw4", dict(eval_fast3=0, eval_pkk_waves=4)),
+                                 ("packed_nw2", dict(eval_fast3=0, eval_pkk_waves=2)),
+                                 ("packed_nw1", dict(eval_fast3=0, eval_pkk_waves=1))):
+                    with G.tuned(**kw):
+                        run = lambda: gb.eval_bcp_dev(d_r, A, d_c, d_o, d_f, d_p)
+                        run()
+                        gb.sync()
+                        e0, e1 = G.Event(), G.Event()
+                        e0.record(gb)
+                        for _ in range(args.steps):
+                            run()
+                        e1.record(gb)
+                        gb.sync()
+                        row[name] = {"kernel": gb.eval_kernel(), "ms": e0.elapsed_ms(e1) / args.steps,
+                                     "props": int(d_p.to_host(np.int32, b.n_inst * A).sum())}
+                res[f"p{pd}"] = row
+                d_r.close(); d_o.close()
+        print(json.dumps(res))
+        return
     if args.general_only:
         import gpupsat_amd as G
         from gpupsat_amd import workloads as W

@@ -332,7 +332,10 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
     // the packed general kernel reads one 8-byte word pair per clause instead of the offsets
     // and the literals (batches that the 3-literal kernel does not take)
     std::vector<uint32_t> h_pkk;
-    const bool use_pkk = packk && !d.fast3 && n_inst > 0 && M > 0;
+    // (GPSAT_PKK_FOR_FAST3=1 builds them for 3-literal batches too, so both kernels can be
+    // compared on one batch with eval_fast3 = 0)
+    const char* pf3 = getenv("GPSAT_PKK_FOR_FAST3");
+    const bool use_pkk = packk && (!d.fast3 || (pf3 && atoi(pf3) != 0)) && n_inst > 0 && M > 0;
     if (use_pkk) {
         h_pkk.resize(2 * M);
         for (int32_t i = 0; i < n_inst; i++) {
