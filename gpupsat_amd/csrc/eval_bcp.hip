@@ -314,6 +314,18 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
 // {T(l), T(l ^ 1)} over those rows, at words [64*W, 128*W + 2).
 // LDS: (128*W + 8) words, so every wavefront's region starts 32-byte aligned (the BCP passes
 // find a slot's partner l ^ 1 by flipping one address bit).
+// Byte address of literal slot FIELD (0-2) of a packed clause word p in a table of
+// (1 << SHIFT)-byte slots at base: one bit-field extract and one shift-add (written out,
+// since the compiler otherwise turns the pair into a shift, a mask and an add).
+template <int FIELD, int SHIFT>
+__device__ __forceinline__ uint32_t slot_addr(uint32_t base, uint32_t p)
+{
+    uint32_t t, r;
+    asm("v_bfe_u32 %0, %1, %2, 10" : "=v"(t) : "v"(p), "i"(10 * FIELD));
+    asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(r) : "v"(t), "i"(SHIFT), "v"(base));
+    return r;
+}
+
 __device__ __forceinline__ int64_t uni64(int64_t x)
 {
     const uint32_t lo = (uint32_t)__builtin_amdgcn_readfirstlane((int)(uint32_t)x);
@@ -486,9 +498,9 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
         // g is evaluated.  Two named buffers alternate, so the unrolled groups need no
         // register copies where a group's scalar guard joins.
         const auto fetch = [&](uint32_t p, u32x4& a, u32x4& b, u32x4& c) {
-            a = dup4[p & 1023u];
-            b = dup4[(p >> 10) & 1023u];
-            c = dup4[p >> 20];
+            a = lds_read16(slot_addr<0, 4>(dupb, p));
+            b = lds_read16(slot_addr<1, 4>(dupb, p));
+            c = lds_read16(slot_addr<2, 4>(dupb, p));
         };
         // per clause: SAT|UNIT = T | (exactly two literals false), UNSAT|UNIT = (at least
         // two false) & ~T, two operations per 32-bit word each
@@ -556,18 +568,20 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                 // kept (and spilled) as masks across the passes
                 int Gp = G;
                 asm volatile("" : "+s"(Gp));
+                // the clause words count as rewritten every pass, so their slot addresses are
+                // formed inside the pass rather than hoisted into 48 live registers
+#pragma unroll
+                for (int g = 0; g < kFastG; g++) asm volatile("" : "+v"(pk[g]));
                 uint64_t chg = 0;       // rows in which this lane's clauses implied a literal
                 // the reads of group g+1 are issued before group g applies its implications,
                 // so they may miss them until the next pass (the fixpoint is unchanged: the
                 // loop ends only after a pass without a single write, or with every row in
-                // conflict).  The empty asm keeps the unpacked literals inside the pass
-                // instead of hoisted into 48 live registers.
+                // conflict).
                 const auto fetch = [&](uint32_t (&s)[3], int g, u32x4& a, u32x4& b, u32x4& c) {
-                    uint32_t p = pk[g];
-                    asm volatile("" : "+v"(p));
-                    s[0] = dupb + 16u * (p & 1023u);
-                    s[1] = dupb + 16u * ((p >> 10) & 1023u);
-                    s[2] = dupb + 16u * (p >> 20);
+                    const uint32_t p = pk[g];
+                    s[0] = slot_addr<0, 4>(dupb, p);
+                    s[1] = slot_addr<1, 4>(dupb, p);
+                    s[2] = slot_addr<2, 4>(dupb, p);
                     a = lds_read16(s[0]);
                     b = lds_read16(s[1]);
                     c = lds_read16(s[2]);
@@ -784,15 +798,17 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                 // kept (and spilled) as masks across the passes
                 int Gp = G;
                 asm volatile("" : "+s"(Gp));
+                // the clause words count as rewritten every pass, so their slot addresses are
+                // formed inside the pass rather than hoisted into 48 live registers
+#pragma unroll
+                for (int g = 0; g < kFastG; g++) asm volatile("" : "+v"(pk[g]));
                 uint32_t chg = 0;       // any implication written by this lane
-                // same two-buffer pipeline as the evaluation; the empty asm keeps each
-                // group's unpacked literals inside the pass.  s[] = the slot addresses.
+                // same two-buffer pipeline as the evaluation.  s[] = the slot addresses.
                 const auto fetch = [&](uint32_t (&s)[3], int g, uint2& a, uint2& b, uint2& c) {
-                    uint32_t p = pk[g];
-                    asm volatile("" : "+v"(p));
-                    s[0] = ctb + 8u * (p & 1023u);
-                    s[1] = ctb + 8u * ((p >> 10) & 1023u);
-                    s[2] = ctb + 8u * (p >> 20);
+                    const uint32_t p = pk[g];
+                    s[0] = slot_addr<0, 3>(ctb, p);
+                    s[1] = slot_addr<1, 3>(ctb, p);
+                    s[2] = slot_addr<2, 3>(ctb, p);
                     a = lds_read8(s[0]);
                     b = lds_read8(s[1]);
                     c = lds_read8(s[2]);
