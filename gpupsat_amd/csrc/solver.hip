@@ -391,6 +391,43 @@ __device__ int32_t analyze(Ctx& J, int32_t confl, int32_t& bt, int32_t& lbd)
     }
     J.lbuf[0] = p ^ 1;
     wave_mem_sync();
+    // Local minimization: a literal whose reason's other literals are all in the clause (or at
+    // level 0) is implied by the rest and dropped.  First every literal is tested against the
+    // clause as learned (drops are marked ~q, seen flags untouched), then the clause is
+    // compacted and the dropped literals' seen flags cleared, so the result does not depend on
+    // lane order.  Dropping several at once is sound: resolving them out in reverse trail
+    // order only meets literals that stay in the clause or are resolved out earlier.
+    if (nl > 1) {
+        for (int32_t k = 1 + lane; k < nl; k += kWave) {
+            const int32_t q = J.lbuf[k];
+            const int32_t v = q >> 1;
+            const int32_t r = J.reason[v];
+            if (r < 0) continue;
+            const int32_t* cl;
+            int32_t len;
+            clause_span(J, r, cl, len);
+            bool drop = true;
+            for (int32_t j = 0; j < len && drop; j++) {
+                const int32_t vx = cl[j] >> 1;
+                if (vx != v && J.seen[vx] == 0 && J.level[vx] > 0) drop = false;
+            }
+            if (drop) J.lbuf[k] = ~q;
+        }
+        wave_mem_sync();
+        int32_t kept = 1;
+        for (int32_t k0 = 1; k0 < nl; k0 += kWave) {
+            const int32_t k = k0 + lane;
+            const int32_t x = k < nl ? J.lbuf[k] : 0;
+            const bool keep = k < nl && x >= 0;
+            if (k < nl && x < 0) J.seen[(~x) >> 1] = 0;
+            const uint64_t km = ballot(keep);
+            wave_mem_sync();
+            if (keep) J.lbuf[kept + prefix_count(km)] = x;
+            kept += popc64(km);
+        }
+        nl = uni(kept);
+        wave_mem_sync();
+    }
     // backjump level = highest level among the other literals; LBD = distinct levels
     J.stamp++;
     uint64_t best = 0;

@@ -386,3 +386,32 @@ def test_restart_schedule(orc, first, factor):
             exact += r == bound
     assert js[:, 5].sum() >= 100
     assert sat >= 20 and exact >= 0.8 * sat, (exact, sat)
+
+
+def test_structured_workload_full_size(orc):
+    """The structured workload at its stated size: 2,000 4-colourings of ER(150, average degree
+    4.2-4.8) and 200 PHP(9,8), one launch.  Exact verdicts: every colouring model verified on the
+    GPU and by the oracle, any UNSAT colouring confirmed by the oracle, every PHP(9,8) UNSAT (the
+    pigeonhole principle); learned-clause counts reported, never above the conflicts."""
+    print("[solve] 2,000 colourings of ER(150) + 200 PHP(9,8)", flush=True)
+    n_col, n_php = 2000, 200
+    col = W.coloring_batch(n_col, seed=4)
+    fs = [col.instance(i) for i in range(n_col)] + [W.pigeonhole_formula(9, 8)] * n_php
+    b = Batch.from_formulas(fs)
+    with G.GpuBatch(b) as gb:
+        res = gb.solve(opts(max_ms=0.0))
+        models = gb.models(res["model_rows"])
+    st, js = res["status"], res["job_stats"]
+    assert (st[n_col:] == G.UNSAT).all()
+    assert (st[:n_col] != G.UNDEF).all()
+    for i in range(n_col):
+        if st[i] == G.SAT:
+            assert res["verified"][i] == 1 and orc.verify_model(b.instance(i), models[i]), i
+        else:
+            assert truth(orc, b.instance(i)) == G.UNSAT, i
+    learned, conflicts = js[:, 4], js[:, 1]
+    assert (learned <= conflicts).all() and (learned >= 0).all()
+    assert (learned[n_col:] > 1000).all()
+    print(f"[solve] colourings SAT {(st[:n_col] == G.SAT).sum()}/{n_col}, learned mean "
+          f"{learned[:n_col].mean():.2f}; PHP(9,8) UNSAT {n_php}/{n_php}, learned mean {learned[n_col:].mean():.0f}, "
+          f"kernel {res['info']['kernel_ms']:.0f} ms", flush=True)
