@@ -952,20 +952,24 @@ __host__ __device__ constexpr int pk_lds_words(int W) { return 128 * W + 8 + 8 +
 
 template <int K>
 struct PkGroup {
-    uint32_t ad[K];     // slot byte addresses
+    uint2 w;            // the lane's clause word pair (slot addresses are formed again from it
+                        // where an implication is written, so they are not held in registers)
     u32x4 q[K];         // slot contents {T lo, T hi, F lo, F hi}
 };
+
+// byte address of slot k of the clause word pair w in the table at tabb
+template <int K>
+__device__ __forceinline__ uint32_t pk_addr(const uint2& w, uint32_t tabb, int k)
+{
+    return tabb + 16u * __builtin_amdgcn_ubfe(k < 3 ? w.x : w.y, 10 * (k % 3), 10);
+}
 
 template <int K>
 __device__ __forceinline__ void pk_fetch(uint2 w, uint32_t tabb, PkGroup<K>& gr)
 {
+    gr.w = w;
 #pragma unroll
-    for (int k = 0; k < K; k++) {
-        const uint32_t word = k < 3 ? w.x : w.y;
-        gr.ad[k] = tabb + 16u * __builtin_amdgcn_ubfe(word, 10 * (k % 3), 10);
-    }
-#pragma unroll
-    for (int k = 0; k < K; k++) gr.q[k] = lds_read16(gr.ad[k]);
+    for (int k = 0; k < K; k++) gr.q[k] = lds_read16(pk_addr<K>(w, tabb, k));
 }
 
 // Per 32-bit half h: T = some literal true, N0 = every literal false, N1 = exactly one
@@ -1003,10 +1007,10 @@ __device__ __forceinline__ void pk_status(const PkGroup<K>& gr, uint32_t (&T)[2]
 }
 
 // One sweep over this wave's clause groups g0, g0 + step, ...: a group's clause words arrive
-// kPkAhead of the wave's groups ahead of their use, and its slot reads are issued before the
-// previous group is processed (they may then miss that group's implications until the next
-// pass; a BCP pass ends the loop only when no wave wrote anything, so the fixpoint is
-// unchanged).  process(group) returns true to stop the sweep.
+// kPkAhead of the wave's groups ahead of their use, and (clauses of up to 4 literals) its slot
+// reads are issued before the previous group is processed (they may then miss that group's
+// implications until the next pass; a BCP pass ends the loop only when no wave wrote
+// anything, so the fixpoint is unchanged).  process(group) returns true to stop the sweep.
 template <int K, class F>
 __device__ __forceinline__ void pk_sweep(const uint2* __restrict__ ipk, int m, int G, int g0, int step, int lane,
                                          uint32_t tabb, uint2 zw, F&& process)
@@ -1019,17 +1023,29 @@ __device__ __forceinline__ void pk_sweep(const uint2* __restrict__ ipk, int m, i
     uint2 pw[kPkAhead];
 #pragma unroll
     for (int u = 0; u < kPkAhead; u++) pw[u] = u < cnt ? load(u) : zw;
-    PkGroup<K> buf[2];
-    if (cnt > 0) pk_fetch<K>(word(pw[0], 0), tabb, buf[0]);
+    // clauses of up to 4 literals: the next group's slot reads go out before this group is
+    // processed (two register sets); longer clauses read their own group's slots (one set,
+    // fewer registers and so more wavefronts per SIMD)
+    constexpr bool kPipe = K <= 4;
+    PkGroup<K> buf[kPipe ? 2 : 1];
+    if constexpr (kPipe) {
+        if (cnt > 0) pk_fetch<K>(word(pw[0], 0), tabb, buf[0]);
+    }
     bool stop = false;
     for (int t0 = 0; t0 < cnt && !stop; t0 += kPkAhead) {
 #pragma unroll
         for (int u = 0; u < kPkAhead; u++) {
             const int t = t0 + u;
             if (t < cnt && !stop) {
-                if (t + 1 < cnt) pk_fetch<K>(word(pw[(u + 1) % kPkAhead], t + 1), tabb, buf[(u + 1) & 1]);
-                if (t + kPkAhead < cnt) pw[u] = load(t + kPkAhead);
-                stop = process(buf[u & 1]);
+                if constexpr (kPipe) {
+                    if (t + 1 < cnt) pk_fetch<K>(word(pw[(u + 1) % kPkAhead], t + 1), tabb, buf[(u + 1) & 1]);
+                    if (t + kPkAhead < cnt) pw[u] = load(t + kPkAhead);
+                    stop = process(buf[u & 1]);
+                } else {
+                    pk_fetch<K>(word(pw[u], t), tabb, buf[0]);
+                    if (t + kPkAhead < cnt) pw[u] = load(t + kPkAhead);
+                    stop = process(buf[0]);
+                }
             }
         }
     }
@@ -1154,8 +1170,9 @@ __device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int
                     for (int k = 0; k < K; k++) {
                         const uint64_t Uk = U & ~u64(gr.q[k].z, gr.q[k].w);
                         if (Uk) {
-                            lds_or64(gr.ad[k], Uk);
-                            lds_or64((gr.ad[k] ^ 16u) + 8u, Uk);
+                            const uint32_t a = pk_addr<K>(gr.w, tabb, k);
+                            lds_or64(a, Uk);
+                            lds_or64((a ^ 16u) + 8u, Uk);
                         }
                     }
                 }
