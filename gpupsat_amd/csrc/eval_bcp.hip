@@ -399,7 +399,7 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                                                     const uint32_t* __restrict__ rows, int A, int32_t* __restrict__ counts,
                                                     uint32_t* __restrict__ out_rows, int32_t* __restrict__ conflict,
                                                     int32_t* __restrict__ props, int32_t* __restrict__ passes_out,
-                                                    int n_inst, int lds_words)
+                                                    int n_inst, int lds_words, int small64)
 {
     extern __shared__ __align__(32) uint32_t lds_all[];
     // the wavefront's slot is uniform: its metadata then comes in with scalar loads
@@ -559,7 +559,9 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
         // compacted table of the rows still open (a handoff).
         bool handoff = false;
         int a_k = 0;            // lane k < n_act: row index of open row k
-        if (n_act > 32) {
+        // (also with at most small64 open rows: compacting them bit by bit costs more than the
+        // 32-bit passes save over the one or two passes such rows usually need)
+        if (n_act > 32 || n_act <= small64) {
             bool changed = true;
             while (changed) {
                 pass++;
@@ -1007,8 +1009,8 @@ __device__ __forceinline__ void pk_status(const PkGroup<K>& gr, uint32_t (&T)[2]
 }
 
 // One sweep over this wave's clause groups g0, g0 + step, ...: a group's clause words arrive
-// kPkAhead of the wave's groups ahead of their use, and (clauses of up to 4 literals) its slot
-// reads are issued before the previous group is processed (they may then miss that group's
+// kPkAhead of the wave's groups ahead of their use, and (3-literal clauses) its slot reads
+// are issued before the previous group is processed (they may then miss that group's
 // implications until the next pass; a BCP pass ends the loop only when no wave wrote
 // anything, so the fixpoint is unchanged).  process(group) returns true to stop the sweep.
 template <int K, class F>
@@ -1023,10 +1025,10 @@ __device__ __forceinline__ void pk_sweep(const uint2* __restrict__ ipk, int m, i
     uint2 pw[kPkAhead];
 #pragma unroll
     for (int u = 0; u < kPkAhead; u++) pw[u] = u < cnt ? load(u) : zw;
-    // clauses of up to 4 literals: the next group's slot reads go out before this group is
-    // processed (two register sets); longer clauses read their own group's slots (one set,
-    // fewer registers and so more wavefronts per SIMD)
-    constexpr bool kPipe = K <= 4;
+    // 3-literal clauses: the next group's slot reads go out before this group is processed
+    // (two register sets); longer clauses read their own group's slots (one set: 70 instead of
+    // 106 VGPRs, 7 wavefronts per SIMD instead of 4)
+    constexpr bool kPipe = K <= 3;
     PkGroup<K> buf[kPipe ? 2 : 1];
     if constexpr (kPipe) {
         if (cnt > 0) pk_fetch<K>(word(pw[0], 0), tabb, buf[0]);
@@ -1342,15 +1344,16 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
         const int wpb = waves_per_block(lds_words);
         dim3 grid((b.n_inst + wpb - 1) / wpb), block(kWave * wpb);
         const size_t lds = (size_t)wpb * lds_words * 4;
+        const int small64 = tuning().eval_fast3_small64;
         if (do_eval && do_bcp)
             hipLaunchKernelGGL((k_eval_bcp3<true, true>), grid, block, lds, st, b.meta, b.pk3, rows, A, counts, out_rows,
-                               conflict, props, passes, b.n_inst, lds_words);
+                               conflict, props, passes, b.n_inst, lds_words, small64);
         else if (do_eval)
             hipLaunchKernelGGL((k_eval_bcp3<true, false>), grid, block, lds, st, b.meta, b.pk3, rows, A, counts,
-                               out_rows, conflict, props, passes, b.n_inst, lds_words);
+                               out_rows, conflict, props, passes, b.n_inst, lds_words, small64);
         else
             hipLaunchKernelGGL((k_eval_bcp3<false, true>), grid, block, lds, st, b.meta, b.pk3, rows, A, counts,
-                               out_rows, conflict, props, passes, b.n_inst, lds_words);
+                               out_rows, conflict, props, passes, b.n_inst, lds_words, small64);
         return hipGetLastError();
     }
     if (ek == EK_PACKED) {

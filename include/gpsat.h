@@ -73,11 +73,16 @@ int gpsat_copy_dev(gpsat_batch *b, void *dst_dev, const void *src_dev, int64_t b
  *   "eval_stage_arena"  1: the general kernel copies each clause arena to LDS first (default 0)
  *   "eval_waves_per_block"  instances (wavefronts) per workgroup of the eval/BCP kernels:
  *                       1, 2 or 4 (default 4)
+ *   "eval_fast3_small64"  3-literal kernel: instances with at most this many rows open after
+ *                       the evaluation run their BCP passes on the 64-bit table instead of a
+ *                       compacted 32-bit one (default 8; 0: always compact)
+ *   "eval_pkk_waves"    wavefronts sharing one instance in the packed general kernel, 1-4 (default 4)
  *   "eval_pkk"          1: evaluate batches of clauses with at most 6 literals and n <= 496
  *                       (that the register-resident kernel does not take) with the packed
  *                       general kernel (default); 0: the CSR general kernel
  * The GPSAT_TUNE_EVAL_FAST3 / GPSAT_TUNE_EVAL_STAGE / GPSAT_TUNE_EVAL_WPB /
- * GPSAT_TUNE_EVAL_PKK environment variables set the initial values.  Returns GPSAT_ERR_ARG
+ * GPSAT_TUNE_EVAL_FAST3_SMALL64 / GPSAT_TUNE_EVAL_PKK / GPSAT_TUNE_EVAL_PKK_WAVES environment
+ * variables set the initial values.  Returns GPSAT_ERR_ARG
  * for an unknown key. */
 int gpsat_tune_set(const char *key, int32_t value);
 int gpsat_tune_get(const char *key, int32_t *value);
