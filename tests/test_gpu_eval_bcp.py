@@ -321,3 +321,16 @@ def test_packed_general_kernel(orc, A, p, nw):
     np.testing.assert_array_equal(props, c_props)
     if p <= 0.05 and A == 64:
         assert props.sum() > 0 and conflict.mean() < 0.9
+
+
+@pytest.mark.parametrize("small64", [0, 8, 32])
+@pytest.mark.parametrize("p", [0.2, 0.3])
+def test_3literal_bcp_modes(orc, p, small64):
+    """The 3-literal kernel's BCP modes for few open rows: compacted bit by bit (small64 = 0),
+    the default (at most 8 open rows stay on the 64-bit table), and the 64-bit table for up to
+    32 open rows; the same oracle fixpoints in every mode."""
+    print(f"[eval/bcp] 3-literal BCP modes p={p} small64={small64}", flush=True)
+    b = W.random_3sat_batch(96, 200, 852, seed=5)
+    rows = W.random_assignments(b, 64, p, seed=int(p * 100) + small64)
+    with G.tuned(eval_fast3=1, eval_fast3_small64=small64):
+        check_batch(orc, b, rows, 64)
