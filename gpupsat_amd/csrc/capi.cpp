@@ -221,20 +221,66 @@ static void pack3_instance(const int32_t* L, int32_t m, uint32_t* out)
 }
 
 // Packed words of one instance for the general path: clause c -> out[2c], out[2c + 1] with
-// slot k's literal code at bit 10 * (k % 3) of word k / 3; slots past the clause's length
-// hold the always-false slot code 32W + 1.
-static void packk_instance(const int32_t* lits, const int64_t* coff, int32_t m, int32_t W, uint32_t* out)
+// slot k's literal code at bit 10 * (k % 3) of word k / 3; slots not holding a literal hold
+// the always-false slot code 32W + 1.  The kernel body reads K = max(3, longest clause) slots
+// per clause, one 16-byte LDS read per slot position across the wavefront.  As for the
+// 3-literal words, the order of the clauses and the slot of each literal change no result,
+// so a greedy pass chooses them: for each 16-lane quarter of a 64-clause group and each slot
+// position it prefers literal slots in distinct residues mod 16 (distinct LDS bank groups),
+// taking candidates from the next 32 unplaced clauses and placing each literal in the free
+// position where its residue is still unused.
+static void packk_instance(const int32_t* lits, const int64_t* coff, int32_t m, int32_t W, int K, uint32_t* out)
 {
     const uint32_t fl = 32u * (uint32_t)W + 1u;
-    for (int32_t c = 0; c < m; c++) {
-        uint32_t w[2] = {0, 0};
-        const int64_t b = coff[c], len = coff[c + 1] - b;
-        for (int k = 0; k < kPkkMaxLen; k++) {
-            const uint32_t code = k < len ? (uint32_t)lits[b + k] : fl;
-            w[k / 3] |= code << (10 * (k % 3));
+    std::vector<char> used((size_t)std::max(m, 1), 0);
+    int32_t head = 0;
+    int pos_best[kPkkMaxLen], pos_try[kPkkMaxLen];
+    for (int32_t b = 0; b < m; b += 16) {
+        uint32_t ur[kPkkMaxLen] = {0, 0, 0, 0, 0, 0};
+        for (int32_t ln = 0; ln < 16 && b + ln < m; ln++) {
+            while (head < m && used[head]) head++;
+            int32_t best = head;
+            int bc = 1 << 30, seen = 0;
+            for (int32_t c = head; c < m && seen < 32 && bc > 0; c++) {
+                if (used[c]) continue;
+                seen++;
+                const int64_t cb = coff[c];
+                const int len = (int)(coff[c + 1] - cb);
+                uint32_t taken = 0;
+                int cost = 0;
+                for (int k = 0; k < len; k++) {
+                    const uint32_t r = (uint32_t)lits[cb + k] & 15u;
+                    int p = -1;
+                    for (int q = 0; q < K && p < 0; q++)
+                        if (!((taken >> q) & 1u) && !((ur[q] >> r) & 1u)) p = q;
+                    if (p < 0) {
+                        for (int q = 0; q < K && p < 0; q++)
+                            if (!((taken >> q) & 1u)) p = q;
+                        cost++;
+                    }
+                    taken |= 1u << p;
+                    pos_try[k] = p;
+                }
+                if (cost < bc) {
+                    bc = cost;
+                    best = c;
+                    for (int k = 0; k < len; k++) pos_best[k] = pos_try[k];
+                }
+            }
+            used[best] = 1;
+            uint32_t code[kPkkMaxLen];
+            for (int k = 0; k < kPkkMaxLen; k++) code[k] = fl;
+            const int64_t cb = coff[best];
+            const int len = (int)(coff[best + 1] - cb);
+            for (int k = 0; k < len; k++) {
+                code[pos_best[k]] = (uint32_t)lits[cb + k];
+                ur[pos_best[k]] |= 1u << ((uint32_t)lits[cb + k] & 15u);
+            }
+            uint32_t w[2] = {0, 0};
+            for (int k = 0; k < kPkkMaxLen; k++) w[k / 3] |= code[k] << (10 * (k % 3));
+            out[2 * (b + ln)] = w[0];
+            out[2 * (b + ln) + 1] = w[1];
         }
-        out[2 * c] = w[0];
-        out[2 * c + 1] = w[1];
     }
 }
 
@@ -340,11 +386,19 @@ int gpsat_batch_create(const int32_t* lits, const int64_t* clause_off, const int
     const bool use_pkk = packk && (!d.fast3 || (pf3 && atoi(pf3) != 0)) && n_inst > 0 && M > 0;
     if (use_pkk) {
         h_pkk.resize(2 * M);
-        for (int32_t i = 0; i < n_inst; i++) {
-            const int64_t cb = inst_clause_off[i];
-            packk_instance(lits, clause_off + cb, d.h_meta[i].m, words_per_row(d.h_meta[i].n),
-                           h_pkk.data() + 2 * (cb - c0));
-        }
+        const int32_t T = (int32_t)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        const int32_t nt = std::min(T, 1 + n_inst / 16);
+        const auto work = [&](int32_t t) {
+            for (int32_t i = t; i < n_inst; i += nt) {
+                const int64_t cb = inst_clause_off[i];
+                packk_instance(lits, clause_off + cb, d.h_meta[i].m, words_per_row(d.h_meta[i].n),
+                               std::max<int>(3, h_klen[i]), h_pkk.data() + 2 * (cb - c0));
+            }
+        };
+        std::vector<std::thread> pool;
+        for (int32_t t = 1; t < nt; t++) pool.emplace_back(work, t);
+        work(0);
+        for (std::thread& th : pool) th.join();
     }
     hipError_t e = hipGetDevice(&d.device);
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&d.stream, hipStreamNonBlocking);
