@@ -161,7 +161,7 @@ def solve_leg(G, W, b, cpu):
     tst = tr["status"]
     leg["trajectory_mode"] = {
         "what": "the reference's sequential search (last-free-var decisions, decision-scheme learning, "
-                "20-clause DB) on the GPU, one instance per lane, no iteration cap",
+                "20-clause DB) on the GPU, one instance per wavefront, no iteration cap",
         "instances_per_s": N / (t_ms * 1e-3), "wall_ms_incl_workspace_alloc": t_ms,
         "decided": int((tst < 2).sum()), "verdicts_equal_cdcl": int(((tst == st) | (tst > 1) | (st > 1)).sum()),
         "capacity_stops": int((tst == 4).sum()),

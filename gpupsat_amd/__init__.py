@@ -360,7 +360,7 @@ class GpuBatch:
 
     # ---- trajectory mode
     def solve_ref(self, dead=None, max_iterations: int = 0, learned_cap: int = 0, learned_lits: int = 0) -> dict:
-        """The reference's own sequential search, one instance per lane (gpsat_solve_ref):
+        """The reference's own sequential search, one instance per wavefront (gpsat_solve_ref):
         status (0 SAT, 1 UNSAT, 2 UNDEF, 4 a graph vertex past the reference's edge capacity),
         model rows and per-instance stats (decisions, conflicts, learned, iterations,
         restarts).  learned_cap / learned_lits are accepted and not used."""

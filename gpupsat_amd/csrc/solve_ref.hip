@@ -809,6 +809,16 @@ __global__ __launch_bounds__(64) void k_solve_ref(RefArgs a)
     if (inst < a.n_inst) solve_ref_instance(a, inst);
 }
 
+// One instance per wavefront, run by its first lane: the searches of different instances
+// diverge at almost every branch, so a wavefront of 64 instances runs them nearly one after
+// another; one per wavefront puts as many independent searches in flight as the GPU holds
+// wavefronts.
+__global__ __launch_bounds__(256) void k_solve_ref_wave(RefArgs a)
+{
+    const int32_t inst = (int32_t)(blockIdx.x * (blockDim.x / 64) + threadIdx.x / 64);
+    if (inst < a.n_inst && (threadIdx.x & 63) == 0) solve_ref_instance(a, inst);
+}
+
 }  // namespace
 
 // Per-instance workspace ints of the trajectory mode (the slices carved in k_solve_ref).
@@ -822,7 +832,10 @@ int64_t solve_ref_ws_ints(int32_t n, int32_t m, int32_t ecap)
 hipError_t launch_solve_ref(const RefArgs& a, hipStream_t st)
 {
     if (a.n_inst == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_solve_ref, dim3((a.n_inst + 63) / 64), dim3(64), 0, st, a);
+    if (tuning().solve_ref_per_wave)
+        hipLaunchKernelGGL(k_solve_ref_wave, dim3((a.n_inst + 3) / 4), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL(k_solve_ref, dim3((a.n_inst + 63) / 64), dim3(64), 0, st, a);
     return hipGetLastError();
 }
 

@@ -276,7 +276,7 @@ int gpsat_solve_noalloc(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t
 
 /* Trajectory mode: the reference's own sequential search per instance, step for step
  * (SATSolver::solve, src/SATSolver/SATSolver.cu:67-218, with the default strategies of
- * Configs.cuh), one instance per GPU lane, over the reference's structures: per-variable watch
+ * Configs.cuh), one instance per wavefront, over the reference's structures: per-variable watch
  * lists with the fewest-watched-clauses choice and the blocked list
  * (BCPStrategy/ClauseListStructure.cu:39-243), depth-first propagation of pending implications
  * (WatchedClausesList.cu:46-309), the implication graph and decision-scheme learning
