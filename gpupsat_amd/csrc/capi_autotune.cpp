@@ -53,7 +53,10 @@ struct Report {
 
 // The sampled job list: whole runs of consecutive jobs of one instance (all the cubes the
 // caller queued for it, so chunked cube reuse is part of what is measured), every s-th run
-// of the caller's list so the sample spreads over the queue, about `target` jobs in all.
+// of the caller's list so the sample spreads over the queue, about `target` jobs in all --
+// or all of them when they are at most twice that many: the end of a run is set by its
+// slowest jobs, so a sample of half the jobs can rank shapes differently from the run itself
+// (on 20,000 n=50 instances a 10,000-job sample preferred a shape 7% slower in full).
 struct Sample {
     std::vector<int32_t> inst, off, lits;
 };
@@ -64,7 +67,8 @@ Sample sample_jobs(const int32_t* job_inst, const int32_t* cube_off, const int32
     Sample s;
     s.off.push_back(0);
     const int32_t total = job_inst ? n_jobs : n_inst;
-    const int64_t stride = std::max<int64_t>(1, (total + target - 1) / std::max<int64_t>(target, 1));
+    const int64_t stride =
+        total <= 2 * target ? 1 : std::max<int64_t>(1, (total + target - 1) / std::max<int64_t>(target, 1));
     if (!job_inst) {
         for (int32_t i = 0; i < n_inst; i += (int32_t)stride) {
             s.inst.push_back(i);
