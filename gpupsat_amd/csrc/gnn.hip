@@ -137,51 +137,65 @@ __global__ __launch_bounds__(256) void k_dense_bwd_dx(const bf16_t* __restrict__
     }
 }
 
+// gfx950's transposing LDS read: within each 16-lane group, lane i receives element i % 4 of
+// the 8-byte chunks that lanes i/4, i/4 + 4, i/4 + 8 and i/4 + 12 address.  With lane L of
+// the group pointing at row L/4, columns 4(L%4) .. 4(L%4)+3 of a row-major 4 x 16 tile, lane
+// i receives column i of those 4 rows.  Two reads (rows r .. r+3, r+4 .. r+7) give the 8
+// consecutive-k values of an MFMA operand from a tile stored with k as its row index.
+typedef short v4s __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ bf16x8 ld8_tr(const bf16_t* p, int row_stride)
+{
+    typedef __attribute__((address_space(3))) v4s lds_v4s;
+    const v4s r0 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s*)p);
+    const v4s r1 = __builtin_amdgcn_ds_read_tr16_b64_v4i16((lds_v4s*)(p + 4 * row_stride));
+    typedef short v8s __attribute__((ext_vector_type(8)));
+    const v8s v = {r0.x, r0.y, r0.z, r0.w, r1.x, r1.y, r1.z, r1.w};
+    return __builtin_bit_cast(bf16x8, v);
+}
+
 // Partial weight gradient of this workgroup: part[blockIdx.x][64 x K] = sum over its
 // 64-node chunks of dP[N x 64]^T X[N x K] (the reduction adds the workgroups' partials in
-// order).  A chunk is staged transposed in LDS (node index contiguous: sPt[h][n], sXt[col][n])
-// by all four wavefronts; wavefront w then computes output rows h = 16w .. 16w+15.
+// order).  A chunk is staged row-major in LDS (node index = row, 16-byte stores, rows padded
+// by 32 bytes) by all four wavefronts, and the MFMA operands, whose k index is the node, come
+// out of it through transposing reads; wavefront w computes output rows h = 16w .. 16w+15.
 template <int K>
 __global__ __launch_bounds__(256) void k_dense_bwd_dw(const bf16_t* __restrict__ dP, const bf16_t* __restrict__ X,
                                                       float* __restrict__ part, int N)
 {
-    constexpr int SN = 64 + 8;
-    __shared__ __align__(16) bf16_t sPt[64 * SN];
-    __shared__ __align__(16) bf16_t sXt[K * SN];
+    constexpr int SP = 64 + 16, SX = K + 16;   // row strides (elements) of the staged tiles
+    __shared__ __align__(16) bf16_t sP[64 * SP];
+    __shared__ __align__(16) bf16_t sX[64 * SX];
     constexpr int CT = K / 16;
     const int t = threadIdx.x, lane = t & 63, wave = t >> 6;
-    const int kq = 8 * (lane >> 4);
+    // this lane's place in a transposing read: row 8(lane/16) + (lane%16)/4 of the k block,
+    // columns 4(lane%4) .. +3 of the 16-column tile
+    const int tr_row = 8 * (lane >> 4) + ((lane & 15) >> 2), tr_col = 4 * (lane & 3);
     f32x4 acc[CT];
 #pragma unroll
     for (int ct = 0; ct < CT; ct++) acc[ct] = f32x4{0.f, 0.f, 0.f, 0.f};
     for (int n0 = blockIdx.x * 64; n0 < N; n0 += gridDim.x * 64) {
         const int rows = min(64, N - n0);
         __syncthreads();
-        for (int i = t; i < 64 * 64 / 4; i += 256) {
-            const int r = (4 * i) / 64, h = (4 * i) % 64;
-            uint2 v = {0u, 0u};
-            if (r < rows) v = reinterpret_cast<const uint2*>(dP + (size_t)n0 * 64)[i];
-            sPt[(h + 0) * SN + r] = (bf16_t)(v.x & 0xFFFFu);
-            sPt[(h + 1) * SN + r] = (bf16_t)(v.x >> 16);
-            sPt[(h + 2) * SN + r] = (bf16_t)(v.y & 0xFFFFu);
-            sPt[(h + 3) * SN + r] = (bf16_t)(v.y >> 16);
+        for (int q = t; q < 64 * 8; q += 256) {
+            const int r = q >> 3, c = (q & 7) * 8;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (r < rows) v = *reinterpret_cast<const uint4*>(dP + (size_t)(n0 + r) * 64 + c);
+            *reinterpret_cast<uint4*>(sP + r * SP + c) = v;
         }
-        for (int i = t; i < 64 * K / 4; i += 256) {
-            const int r = (4 * i) / K, c = (4 * i) % K;
-            uint2 v = {0u, 0u};
-            if (r < rows) v = reinterpret_cast<const uint2*>(X + (size_t)n0 * K)[i];
-            sXt[(c + 0) * SN + r] = (bf16_t)(v.x & 0xFFFFu);
-            sXt[(c + 1) * SN + r] = (bf16_t)(v.x >> 16);
-            sXt[(c + 2) * SN + r] = (bf16_t)(v.y & 0xFFFFu);
-            sXt[(c + 3) * SN + r] = (bf16_t)(v.y >> 16);
+        for (int q = t; q < 64 * (K / 8); q += 256) {
+            const int r = q / (K / 8), c = (q % (K / 8)) * 8;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (r < rows) v = *reinterpret_cast<const uint4*>(X + (size_t)(n0 + r) * K + c);
+            *reinterpret_cast<uint4*>(sX + r * SX + c) = v;
         }
         __syncthreads();
 #pragma unroll
         for (int s = 0; s < 64; s += 32) {
             // A[h][n] = dP[n][h] (16 h x 32 n), B[n][col] = X[n][col] (32 n x 16 col)
-            const bf16x8 a = ld8(sPt + (wave * 16 + (lane & 15)) * SN + s + kq);
+            const bf16x8 a = ld8_tr(sP + (s + tr_row) * SP + 16 * wave + tr_col, SP);
 #pragma unroll
-            for (int ct = 0; ct < CT; ct++) acc[ct] = mfma32(a, ld8(sXt + (ct * 16 + (lane & 15)) * SN + s + kq), acc[ct]);
+            for (int ct = 0; ct < CT; ct++)
+                acc[ct] = mfma32(a, ld8_tr(sX + (s + tr_row) * SX + 16 * ct + tr_col, SX), acc[ct]);
         }
     }
     float* out = part + (size_t)blockIdx.x * 64 * K;
