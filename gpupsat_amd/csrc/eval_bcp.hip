@@ -575,10 +575,12 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
 #pragma unroll
                 for (int g = 0; g < kFastG; g++) asm volatile("" : "+v"(pk[g]));
                 uint64_t chg = 0;       // rows in which this lane's clauses implied a literal
-                // the reads of group g+1 are issued before group g applies its implications,
-                // so they may miss them until the next pass (the fixpoint is unchanged: the
-                // loop ends only after a pass without a single write, or with every row in
-                // conflict).
+                // Group g's reads are issued after group g-1's implication writes (LDS
+                // operations of one wavefront complete in order), so every implication is seen
+                // by the groups after it in the same pass.  Reading group g+1 ahead of group g's
+                // writes hides the read latency within the wavefront but misses those
+                // implications until the next pass: ~8% more passes at p = 0.1, and the other
+                // wavefronts of the SIMD hide the latency anyway.
                 const auto fetch = [&](uint32_t (&s)[3], int g, u32x4& a, u32x4& b, u32x4& c) {
                     const uint32_t p = pk[g];
                     s[0] = slot_addr<0, 4>(dupb, p);
@@ -616,18 +618,13 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                     }
                     return false;
                 };
-                uint32_t px[3], py[3];
-                u32x4 xa, xb, xc, ya, yb, yc;
-                fetch(px, 0, xa, xb, xc);
+                uint32_t s3[3];
+                u32x4 qa, qb, qc;
 #pragma unroll
-                for (int g = 0; g < kFastG; g += 2) {
+                for (int g = 0; g < kFastG; g++) {
                     if (g < Gp) {
-                        if (g + 1 < kFastG) fetch(py, g + 1, ya, yb, yc);
-                        if (group(px, xa, xb, xc)) break;
-                    }
-                    if (g + 1 < Gp) {
-                        if (g + 2 < kFastG) fetch(px, g + 2, xa, xb, xc);
-                        if (group(py, ya, yb, yc)) break;
+                        fetch(s3, g, qa, qb, qc);
+                        if (group(s3, qa, qb, qc)) break;
                     }
                 }
                 const uint64_t live = wave_or64(chg) & amask & ~frozen;   // rows still moving
@@ -805,7 +802,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
 #pragma unroll
                 for (int g = 0; g < kFastG; g++) asm volatile("" : "+v"(pk[g]));
                 uint32_t chg = 0;       // any implication written by this lane
-                // same two-buffer pipeline as the evaluation.  s[] = the slot addresses.
+                // groups in order, each read after the previous group's writes (as in the
+                // 64-bit passes).  s[] = the slot addresses.
                 const auto fetch = [&](uint32_t (&s)[3], int g, uint2& a, uint2& b, uint2& c) {
                     const uint32_t p = pk[g];
                     s[0] = slot_addr<0, 3>(ctb, p);
@@ -844,18 +842,13 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                     }
                     return false;
                 };
-                uint32_t px[3], py[3];
-                uint2 xa, xb, xc, ya, yb, yc;
-                fetch(px, 0, xa, xb, xc);
+                uint32_t s3[3];
+                uint2 qa, qb, qc;
 #pragma unroll
-                for (int g = 0; g < kFastG; g += 2) {
+                for (int g = 0; g < kFastG; g++) {
                     if (g < Gp) {
-                        if (g + 1 < kFastG) fetch(py, g + 1, ya, yb, yc);
-                        if (group(px, xa, xb, xc)) break;
-                    }
-                    if (g + 1 < Gp) {
-                        if (g + 2 < kFastG) fetch(px, g + 2, xa, xb, xc);
-                        if (group(py, ya, yb, yc)) break;
+                        fetch(s3, g, qa, qb, qc);
+                        if (group(s3, qa, qb, qc)) break;
                     }
                 }
                 changed = ballot(chg != 0) != 0ull && (fz & amask32) != amask32;
@@ -1009,11 +1002,13 @@ __device__ __forceinline__ void pk_status(const PkGroup<K>& gr, uint32_t (&T)[2]
 }
 
 // One sweep over this wave's clause groups g0, g0 + step, ...: a group's clause words arrive
-// kPkAhead of the wave's groups ahead of their use, and (3-literal clauses) its slot reads
-// are issued before the previous group is processed (they may then miss that group's
-// implications until the next pass; a BCP pass ends the loop only when no wave wrote
-// anything, so the fixpoint is unchanged).  process(group) returns true to stop the sweep.
-template <int K, class F>
+// kPkAhead of the wave's groups ahead of their use.  With PIPE (the evaluation, which writes
+// nothing to the table) and 3-literal clauses, a group's slot reads are issued before the
+// previous group is processed.  BCP passes read a group's slots only after the previous
+// group's implication writes, so a pass sees every implication made before it in the same
+// wavefront (reading ahead would miss them until the next pass: 9-10% more time at p = 0.05
+// and 0.1 on mixed k-SAT).  process(group) returns true to stop the sweep.
+template <int K, bool PIPE, class F>
 __device__ __forceinline__ void pk_sweep(const uint2* __restrict__ ipk, int m, int G, int g0, int step, int lane,
                                          uint32_t tabb, uint2 zw, F&& process)
 {
@@ -1025,10 +1020,10 @@ __device__ __forceinline__ void pk_sweep(const uint2* __restrict__ ipk, int m, i
     uint2 pw[kPkAhead];
 #pragma unroll
     for (int u = 0; u < kPkAhead; u++) pw[u] = u < cnt ? load(u) : zw;
-    // 3-literal clauses: the next group's slot reads go out before this group is processed
-    // (two register sets); longer clauses read their own group's slots (one set: 70 instead of
-    // 106 VGPRs, 7 wavefronts per SIMD instead of 4)
-    constexpr bool kPipe = K <= 3;
+    // 3-literal clauses in the evaluation: the next group's slot reads go out before this
+    // group is processed (two register sets); longer clauses read their own group's slots (one
+    // set: 70 instead of 106 VGPRs, 7 wavefronts per SIMD instead of 4)
+    constexpr bool kPipe = PIPE && K <= 3;
     PkGroup<K> buf[kPipe ? 2 : 1];
     if constexpr (kPipe) {
         if (cnt > 0) pk_fetch<K>(word(pw[0], 0), tabb, buf[0]);
@@ -1112,7 +1107,7 @@ __device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int
     uint64_t unsat_any = 0;
     if (DO_EVAL) {
         uint32_t acc[2][3] = {{0, 0, 0}, {0, 0, 0}};
-        pk_sweep<K>(ipk, m, G, wv, NW, lane, tabb, zw, [&](const PkGroup<K>& gr) {
+        pk_sweep<K, true>(ipk, m, G, wv, NW, lane, tabb, zw, [&](const PkGroup<K>& gr) {
             uint32_t T[2], N0[2], N1[2];
             pk_status<K>(gr, T, N0, N1);
             // SAT|UNIT and UNSAT|UNIT per clause
@@ -1160,7 +1155,7 @@ __device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int
             pass++;
             frozen |= lds_ld64(sh_frozen);
             uint64_t chg = 0;       // rows in which this lane's clauses implied a literal
-            if ((frozen & amask) != amask) pk_sweep<K>(ipk, m, G, wv, NW, lane, tabb, zw, [&](const PkGroup<K>& gr) {
+            if ((frozen & amask) != amask) pk_sweep<K, false>(ipk, m, G, wv, NW, lane, tabb, zw, [&](const PkGroup<K>& gr) {
                 uint32_t T[2], N0[2], N1[2];
                 pk_status<K>(gr, T, N0, N1);
                 const uint64_t open = ~(u64(T[0], T[1]) | frozen);
