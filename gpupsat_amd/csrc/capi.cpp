@@ -25,6 +25,7 @@ Tuning& tuning()
         if (const char* e = getenv("GPSAT_TUNE_EVAL_WPB")) x.eval_waves_per_block = atoi(e);
         if (const char* e = getenv("GPSAT_TUNE_EVAL_PKK")) x.eval_pkk = atoi(e);
         if (const char* e = getenv("GPSAT_TUNE_EVAL_PKK_WAVES")) x.eval_pkk_waves = atoi(e);
+        if (const char* e = getenv("GPSAT_TUNE_EVAL_PKK_COMPACT")) x.eval_pkk_compact = atoi(e);
         if (const char* e = getenv("GPSAT_TUNE_SOLVE_REF_PER_WAVE")) x.solve_ref_per_wave = atoi(e);
         if (const char* e = getenv("GPSAT_TUNE_EVAL_FAST3_SMALL64")) x.eval_fast3_small64 = atoi(e);
         return x;
@@ -65,6 +66,7 @@ static int* tune_slot(const char* key)
     if (!strcmp(key, "eval_waves_per_block")) return &tuning().eval_waves_per_block;
     if (!strcmp(key, "eval_pkk")) return &tuning().eval_pkk;
     if (!strcmp(key, "eval_pkk_waves")) return &tuning().eval_pkk_waves;
+    if (!strcmp(key, "eval_pkk_compact")) return &tuning().eval_pkk_compact;
     if (!strcmp(key, "solve_ref_per_wave")) return &tuning().solve_ref_per_wave;
     if (!strcmp(key, "eval_fast3_small64")) return &tuning().eval_fast3_small64;
     return nullptr;

@@ -935,7 +935,9 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
 // instance is swept NW times faster than by one wave.  Implications go straight into the
 // shared table; each BCP pass ends at one barrier, where the waves' "implied something"
 // rows and conflict rows are combined in LDS.  The table is built straight from the input
-// rows in HBM and the closure rows go straight back.
+// rows in HBM and the closure rows go straight back.  On long instances, once at most 32 rows
+// still move, the rows at their fixpoint are written out and the passes go on over a table of
+// 8-byte slots {T, F} holding 32-bit masks of the moving rows, built in place of the first.
 //
 // LDS per workgroup (words): the table, 128W + 8; then the conflict rows (2), three
 // "implied something" words of 2 (pass p uses slot p % 3: the slot the next pass ORs into
@@ -965,6 +967,48 @@ __device__ __forceinline__ void pk_fetch(uint2 w, uint32_t tabb, PkGroup<K>& gr)
     gr.w = w;
 #pragma unroll
     for (int k = 0; k < K; k++) gr.q[k] = lds_read16(pk_addr<K>(w, tabb, k));
+}
+
+// A group's slots in the compacted table of at most 32 open rows: 8-byte slots {T, F}.
+template <int K>
+struct PkGroup32 {
+    uint2 w;
+    uint2 q[K];
+};
+
+template <int K>
+__device__ __forceinline__ uint32_t pk_addr8(const uint2& w, uint32_t tabb, int k)
+{
+    return tabb + 8u * __builtin_amdgcn_ubfe(k < 3 ? w.x : w.y, 10 * (k % 3), 10);
+}
+
+template <int K>
+__device__ __forceinline__ void pk_fetch(uint2 w, uint32_t tabb, PkGroup32<K>& gr)
+{
+    gr.w = w;
+#pragma unroll
+    for (int k = 0; k < K; k++) gr.q[k] = lds_read8(pk_addr8<K>(w, tabb, k));
+}
+
+// The same over the compacted rows: one 32-bit word each.
+template <int K>
+__device__ __forceinline__ void pk_status32(const PkGroup32<K>& gr, uint32_t& T, uint32_t& N0, uint32_t& N1)
+{
+    if (K == 3) {
+        T = gr.q[0].x | gr.q[1].x | gr.q[2].x;
+        N0 = and3(gr.q[0].y, gr.q[1].y, gr.q[2].y);
+        N1 = two3(gr.q[0].y, gr.q[1].y, gr.q[2].y);
+        return;
+    }
+    T = gr.q[0].x;
+    N0 = gr.q[0].y;
+    N1 = ~gr.q[0].y;
+#pragma unroll
+    for (int k = 1; k < K; k++) {
+        T |= gr.q[k].x;
+        N1 = bfi(gr.q[k].y, N1, N0);
+        N0 &= gr.q[k].y;
+    }
 }
 
 // Per 32-bit half h: T = some literal true, N0 = every literal false, N1 = exactly one
@@ -1008,7 +1052,7 @@ __device__ __forceinline__ void pk_status(const PkGroup<K>& gr, uint32_t (&T)[2]
 // group's implication writes, so a pass sees every implication made before it in the same
 // wavefront (reading ahead would miss them until the next pass: 9-10% more time at p = 0.05
 // and 0.1 on mixed k-SAT).  process(group) returns true to stop the sweep.
-template <int K, bool PIPE, class F>
+template <int K, bool PIPE, class Grp = PkGroup<K>, class F>
 __device__ __forceinline__ void pk_sweep(const uint2* __restrict__ ipk, int m, int G, int g0, int step, int lane,
                                          uint32_t tabb, uint2 zw, F&& process)
 {
@@ -1024,7 +1068,7 @@ __device__ __forceinline__ void pk_sweep(const uint2* __restrict__ ipk, int m, i
     // group is processed (two register sets); longer clauses read their own group's slots (one
     // set: 70 instead of 106 VGPRs, 7 wavefronts per SIMD instead of 4)
     constexpr bool kPipe = PIPE && K <= 3;
-    PkGroup<K> buf[kPipe ? 2 : 1];
+    Grp buf[kPipe ? 2 : 1];
     if constexpr (kPipe) {
         if (cnt > 0) pk_fetch<K>(word(pw[0], 0), tabb, buf[0]);
     }
@@ -1062,7 +1106,7 @@ __device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int
                                                const uint2* __restrict__ pkk, const uint32_t* __restrict__ rows, int A,
                                                int32_t* __restrict__ counts, uint32_t* __restrict__ out_rows,
                                                int32_t* __restrict__ conflict, int32_t* __restrict__ props,
-                                               int32_t* __restrict__ passes_out)
+                                               int32_t* __restrict__ passes_out, int compact_max)
 {
     const int W = __builtin_amdgcn_readfirstlane(words_per_row(mi.n));
     const int m = __builtin_amdgcn_readfirstlane(mi.m);
@@ -1151,6 +1195,7 @@ __device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int
         // conflict rows only grow, and any snapshot of them is a valid one.
         uint64_t frozen = 0;
         int pass = 0;
+        uint64_t mv = 0;            // rows the last pass moved (the same in every wave)
         for (;;) {
             pass++;
             frozen |= lds_ld64(sh_frozen);
@@ -1190,7 +1235,16 @@ __device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int
                                                 __HIP_MEMORY_SCOPE_WORKGROUP);
             }
             __syncthreads();
-            if ((lds_ld64(&sh_chg[pass % 3]) & amask) == 0ull) break;
+            mv = lds_ld64(&sh_chg[pass % 3]) & amask;
+            if (mv == 0ull) break;
+            // From the second pass on, with at most compact_max rows still moving, the passes go
+            // on over a table of 32-bit masks of those rows (a handoff) when a pass costs each
+            // wave at least about 4W slot reads (G K / NW >= 4 W): the handoff itself is ~2W
+            // transposes on one wave, and it pays back over the next few passes only on long
+            // instances (on the mixed k-SAT workload, the 5-SAT ones: -6% at p = 0.3, no
+            // change at p = 0.02-0.1; handing off after the first pass, or for shorter
+            // instances, cost up to 9% at p = 0.05)
+            if (pass >= 2 && __builtin_popcountll(mv) <= compact_max && G * K >= 4 * W * NW) break;
         }
         // complementary implications are conflicts too
         {
@@ -1203,8 +1257,130 @@ __device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int
             if (lane == 0 && cm) lds_or64p(sh_frozen, cm);
         }
         __syncthreads();
-        if (wv != 0) return;
+        // no wave writes the conflict rows from here on: every wave reads the same ones
         frozen = lds_ld64(sh_frozen);
+        const uint64_t live = mv & ~frozen;     // rows for the compacted passes
+        if (live != 0ull) {
+            // Handoff.  Wave 0 reads the rows back from the 64-bit table word by word (word
+            // j's slots lie at table words [128j, 128j + 128)), writes the output of every row
+            // at its fixpoint or in conflict, and builds the compacted table of the live rows
+            // in place (word j's 8-byte slots go to [64j, 64j + 64), below every 64-bit slot
+            // still to be read).  Open row k of the compacted table is row a_k.
+            const int n_live = __builtin_popcountll(live);
+            const uint32_t amask32 = n_live >= 32 ? ~0u : ((1u << n_live) - 1u);
+            uint32_t* sh_map = sh_tot;                  // [32]: row of open row k
+            uint32_t* sh_fz = sh_tot + 32;              // open rows in conflict
+            uint32_t* ct = tab;
+            const uint32_t ctb = tabb;
+            const uint2* ct2 = reinterpret_cast<const uint2*>(ct);
+            const bool me_live = (live >> lane) & 1ull;
+            int a_k = 0;
+            int dcnt = 0;                               // lane a: implied literals of done row a
+            if (wv == 0) {
+                if (me_live) sh_map[__builtin_popcountll(live & ((1ull << lane) - 1ull))] = (uint32_t)lane;
+                if (lane == 0) *sh_fz = 0u;
+                wave_sync();
+                a_k = lane < n_live ? (int)sh_map[lane] : 0;
+                const bool my_conf = (frozen >> lane) & 1ull;
+                uint32_t* orow = out_rows + row0;
+                for (int j = 0; j < W; j++) {
+                    const uint32_t x = tp(tab[4 * (32 * j + i) + half]);   // lane a: word j of row a
+                    if (lane < A && !me_live) {
+                        const uint32_t in = src[lane * W + j];
+                        const uint32_t o = my_conf ? in : x;
+                        dcnt += __builtin_popcount(o) - __builtin_popcount(in);
+                        orow[lane * W + j] = o;
+                    }
+                    uint32_t xk = (uint32_t)__builtin_amdgcn_ds_bpermute(a_k << 2, (int)x);
+                    xk = tp(lane < n_live ? xk : 0u);                       // lane l < 32: literal 32j + l
+                    if (lane < 32) {
+                        const int l = 32 * j + lane;
+                        ct[2 * l] = xk;
+                        ct[2 * (l ^ 1) + 1] = xk;
+                    }
+                }
+                // the zero slot, then the always-false slot
+                if (lane < 4) ct[64 * W + lane] = lane < 3 ? 0u : ~0u;
+            }
+            __syncthreads();
+            uint32_t fz = 0u;
+            for (;;) {
+                pass++;
+                fz |= __hip_atomic_load(sh_fz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                uint32_t chg = 0;
+                if ((fz & amask32) != amask32)
+                    pk_sweep<K, false, PkGroup32<K>>(ipk, m, G, wv, NW, lane, ctb, zw, [&](const PkGroup32<K>& gr) {
+                        uint32_t T, N0, N1;
+                        pk_status32<K>(gr, T, N0, N1);
+                        const uint32_t open = ~(T | fz);
+                        const uint32_t newconf = N0 & open;
+                        const uint32_t U = N1 & open;
+                        if (U) {
+#pragma unroll
+                            for (int k = 0; k < K; k++) {
+                                const uint32_t Uk = U & ~gr.q[k].y;
+                                if (Uk) {
+                                    const uint32_t a = pk_addr8<K>(gr.w, ctb, k);
+                                    lds_or32(a, Uk);
+                                    lds_or32((a ^ 8u) + 4u, Uk);
+                                }
+                            }
+                        }
+                        chg |= U;
+                        if (ballot(newconf != 0u)) {
+                            fz |= wave_or32(newconf);
+                            return (fz & amask32) == amask32;
+                        }
+                        return false;
+                    });
+                const uint32_t moved = wave_or32(chg) & ~fz;
+                if (lane == 0) {
+                    if (fz) lds_or32(lds_addr(sh_fz), fz);
+                    if (moved) lds_or64p(&sh_chg[pass % 3], moved);
+                    if (wv == 0) __hip_atomic_store((unsigned long long*)&sh_chg[(pass + 1) % 3], 0ull,
+                                                    __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+                }
+                __syncthreads();
+                if ((lds_ld64(&sh_chg[pass % 3]) & amask32) == 0ull) break;
+            }
+            // complementary implications are conflicts too
+            {
+                uint32_t c = 0u;
+                for (int v = tid; v < 16 * W; v += nthreads) {
+                    const uint2 q = ct2[2 * v];
+                    c |= q.x & q.y;
+                }
+                c = wave_or32(c) | fz;
+                if (lane == 0 && c) lds_or32(lds_addr(sh_fz), c);
+            }
+            __syncthreads();
+            if (wv != 0) return;
+            fz = __hip_atomic_load(sh_fz, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+            // lane k < n_live: open row k, transposed back from the compacted table
+            const bool conf_k = lane < n_live && ((fz >> lane) & 1u);
+            uint32_t* orow = out_rows + row0;
+            int cnt_k = 0;
+            for (int j = 0; j < W; j++) {
+                const uint32_t x = tp(lane < 32 ? ct[2 * (32 * j + lane)] : 0u);   // lane k: word j of open row k
+                if (lane < n_live) {
+                    const uint32_t in = src[a_k * W + j];
+                    const uint32_t o = conf_k ? in : x;
+                    cnt_k += __builtin_popcount(o) - __builtin_popcount(in);
+                    orow[a_k * W + j] = o;
+                }
+            }
+            const int k_me = __builtin_popcountll(live & ((1ull << lane) - 1ull));
+            const int cnt_open = __shfl(cnt_k, k_me);
+            const bool conf_open = __shfl((int)conf_k, k_me) != 0;
+            if (lane < A) {
+                const bool c = me_live ? conf_open : ((frozen >> lane) & 1ull) != 0ull;
+                conflict[(int64_t)inst * A + lane] = c ? 1 : 0;
+                props[(int64_t)inst * A + lane] = me_live ? cnt_open : dcnt;
+            }
+            if (passes_out && lane == 0) passes_out[inst] = pass;
+            return;
+        }
+        if (wv != 0) return;
         const bool my_conflict = (frozen >> lane) & 1ull;
         const bool all_conflict = (frozen & amask) == amask;
         uint32_t* orow = out_rows + row0;
@@ -1244,7 +1420,7 @@ __global__ __launch_bounds__(256) void k_eval_bcpk(const InstMeta* __restrict__ 
                                                     const uint32_t* __restrict__ rows, int A,
                                                     int32_t* __restrict__ counts, uint32_t* __restrict__ out_rows,
                                                     int32_t* __restrict__ conflict, int32_t* __restrict__ props,
-                                                    int32_t* __restrict__ passes_out)
+                                                    int32_t* __restrict__ passes_out, int compact_max)
 {
     extern __shared__ __align__(32) uint32_t lds_all[];
     const int NW = blockDim.x / kWave;
@@ -1255,16 +1431,16 @@ __global__ __launch_bounds__(256) void k_eval_bcpk(const InstMeta* __restrict__ 
     const int K = __builtin_amdgcn_readfirstlane((int)klen[inst]);
     if (K <= 3)
         eval_bcpk_body<3, DO_EVAL, DO_BCP>(mi, inst, wv, NW, lane, lds_all, pkk, rows, A, counts, out_rows, conflict,
-                                           props, passes_out);
+                                           props, passes_out, compact_max);
     else if (K == 4)
         eval_bcpk_body<4, DO_EVAL, DO_BCP>(mi, inst, wv, NW, lane, lds_all, pkk, rows, A, counts, out_rows, conflict,
-                                           props, passes_out);
+                                           props, passes_out, compact_max);
     else if (K == 5)
         eval_bcpk_body<5, DO_EVAL, DO_BCP>(mi, inst, wv, NW, lane, lds_all, pkk, rows, A, counts, out_rows, conflict,
-                                           props, passes_out);
+                                           props, passes_out, compact_max);
     else
         eval_bcpk_body<6, DO_EVAL, DO_BCP>(mi, inst, wv, NW, lane, lds_all, pkk, rows, A, counts, out_rows, conflict,
-                                           props, passes_out);
+                                           props, passes_out, compact_max);
 }
 
 // LDS words of one wavefront's region in the general kernel (literal masks, row staging,
@@ -1357,15 +1533,16 @@ hipError_t launch_eval_bcp(const DeviceBatch& b, const uint32_t* rows, int A, in
         dim3 grid(b.n_inst), block(kWave * nw);
         const size_t lds = (size_t)pk_lds_words(words_per_row(b.max_n)) * 4;
         const uint2* pk = reinterpret_cast<const uint2*>(b.pkk);
+        const int cmax = std::max(0, std::min(32, tuning().eval_pkk_compact));
         if (do_eval && do_bcp)
             hipLaunchKernelGGL((k_eval_bcpk<true, true>), grid, block, lds, st, b.meta, pk, b.klen, b.korder, rows, A,
-                               counts, out_rows, conflict, props, passes);
+                               counts, out_rows, conflict, props, passes, cmax);
         else if (do_eval)
             hipLaunchKernelGGL((k_eval_bcpk<true, false>), grid, block, lds, st, b.meta, pk, b.klen, b.korder, rows, A,
-                               counts, out_rows, conflict, props, passes);
+                               counts, out_rows, conflict, props, passes, cmax);
         else
             hipLaunchKernelGGL((k_eval_bcpk<false, true>), grid, block, lds, st, b.meta, pk, b.klen, b.korder, rows, A,
-                               counts, out_rows, conflict, props, passes);
+                               counts, out_rows, conflict, props, passes, cmax);
         return hipGetLastError();
     }
     // stage the arena in LDS when the largest instance keeps the wavefront under 40 KB

@@ -53,6 +53,9 @@ struct Tuning {
     int eval_waves_per_block = 4; // independent wavefronts (instances) per workgroup: 1, 2 or 4
     int eval_pkk = 1;             // use the packed general kernel when the batch allows it
     int eval_pkk_waves = 4;       // wavefronts sharing one instance in the packed kernel: 1-4
+    int eval_pkk_compact = 32;    // packed kernel: once at most this many rows (0-32) are still
+                                  // moving after a BCP pass, the passes go on over a table of
+                                  // 32-bit masks of those rows (0: never)
     int solve_ref_per_wave = 1;   // trajectory mode: one instance per wavefront (0: per lane)
     int eval_fast3_small64 = 8;   // 3-literal kernel: at most this many rows open after the
                                   // evaluation run on the 64-bit table instead of a compacted one

@@ -77,12 +77,15 @@ int gpsat_copy_dev(gpsat_batch *b, void *dst_dev, const void *src_dev, int64_t b
  *                       the evaluation run their BCP passes on the 64-bit table instead of a
  *                       compacted 32-bit one (default 8; 0: always compact)
  *   "eval_pkk_waves"    wavefronts sharing one instance in the packed general kernel, 1-4 (default 4)
+ *   "eval_pkk_compact"  packed general kernel: from the second BCP pass on, long instances with
+ *                       at most this many rows still moving continue on 32-bit masks of those
+ *                       rows, 0-32 (default 32; 0: never)
  *   "eval_pkk"          1: evaluate batches of clauses with at most 6 literals and n <= 496
  *                       (that the register-resident kernel does not take) with the packed
  *                       general kernel (default); 0: the CSR general kernel
  * The GPSAT_TUNE_EVAL_FAST3 / GPSAT_TUNE_EVAL_STAGE / GPSAT_TUNE_EVAL_WPB /
- * GPSAT_TUNE_EVAL_FAST3_SMALL64 / GPSAT_TUNE_EVAL_PKK / GPSAT_TUNE_EVAL_PKK_WAVES environment
- * variables set the initial values.  Returns GPSAT_ERR_ARG
+ * GPSAT_TUNE_EVAL_FAST3_SMALL64 / GPSAT_TUNE_EVAL_PKK / GPSAT_TUNE_EVAL_PKK_WAVES /
+ * GPSAT_TUNE_EVAL_PKK_COMPACT environment variables set the initial values.  Returns GPSAT_ERR_ARG
  * for an unknown key. */
 int gpsat_tune_set(const char *key, int32_t value);
 int gpsat_tune_get(const char *key, int32_t *value);

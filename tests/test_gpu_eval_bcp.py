@@ -323,6 +323,25 @@ def test_packed_general_kernel(orc, A, p, nw):
         assert props.sum() > 0 and conflict.mean() < 0.9
 
 
+@pytest.mark.parametrize("cmax,nw", [(0, 4), (8, 4), (32, 4), (32, 1), (32, 3)])
+@pytest.mark.parametrize("p", [0.1, 0.3])
+def test_packed_compacted_passes(orc, p, cmax, nw):
+    """The packed kernel's handoff to 32-bit masks once at most cmax rows still move (0: never):
+    on the mixed test batch and on 48 instances of the mixed k-SAT workload (k 3/4/5, n up to
+    400, where 5-SAT rows at p = 0.3 keep moving for many passes), the oracle's counts, closure
+    rows, conflicts and props bit for bit."""
+    print(f"[eval/bcp] packed compacted passes p={p} cmax={cmax} waves={nw}", flush=True)
+    mixed, ks = W.mixed_ksat_batch(4096, seed=3)
+    idx = [int(i) for k in (3, 4, 5) for i in np.nonzero(ks == k)[0][:16]]
+    pb = packed_batch()
+    b = Batch.from_formulas([pb.instance(i) for i in range(pb.n_inst)] + [mixed.instance(i) for i in idx])
+    rows = packed_rows(b, 64, p, seed=int(100 * p) + cmax + nw)
+    with G.tuned(eval_pkk_compact=cmax, eval_pkk_waves=nw):
+        with G.GpuBatch(b) as gb:
+            assert gb.eval_kernel() == "packed"
+        check_batch(orc, b, rows, 64)
+
+
 @pytest.mark.parametrize("small64", [0, 8, 32])
 @pytest.mark.parametrize("p", [0.2, 0.3])
 def test_3literal_bcp_modes(orc, p, small64):
