@@ -33,6 +33,7 @@ struct ScratchMem {
         used += bytes;
         return hipSuccess;
     }
+    void mark_synced() {}
     static constexpr bool kNoAlloc = true;
     size_t remaining() const { return size - used; }
 };

@@ -5,6 +5,8 @@
 #include <string.h>
 
 #include <algorithm>
+#include <map>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -16,6 +18,59 @@
 using namespace gpsat;
 
 namespace gpsat {
+
+namespace {
+std::mutex g_cache_mu;
+std::map<int, std::map<size_t, std::vector<void*>>> g_cache;   // device -> size -> free blocks
+std::map<int, size_t> g_cache_bytes;
+constexpr size_t kCacheMaxBytes = (size_t)1 << 30;
+std::map<int, hipDeviceProp_t> g_props;
+}  // namespace
+
+size_t devmem_round(size_t bytes)
+{
+    if (bytes <= ((size_t)1 << 20)) return (bytes + 4095) & ~(size_t)4095;
+    size_t p2 = (size_t)1 << 20;
+    while (p2 < bytes) p2 <<= 1;
+    const size_t step = p2 / 8;
+    return (bytes + step - 1) / step * step;
+}
+
+void* devmem_cache_take(int dev, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto& m = g_cache[dev];
+    auto it = m.find(bytes);
+    if (it == m.end() || it->second.empty()) return nullptr;
+    void* p = it->second.back();
+    it->second.pop_back();
+    g_cache_bytes[dev] -= bytes;
+    return p;
+}
+
+bool devmem_cache_give(int dev, void* p, size_t bytes)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    if (g_cache_bytes[dev] + bytes > kCacheMaxBytes) return false;
+    g_cache[dev][bytes].push_back(p);
+    g_cache_bytes[dev] += bytes;
+    return true;
+}
+
+// Device properties, queried once per device.
+static hipError_t device_props(int dev, hipDeviceProp_t& out)
+{
+    std::lock_guard<std::mutex> lk(g_cache_mu);
+    auto it = g_props.find(dev);
+    if (it == g_props.end()) {
+        hipDeviceProp_t prop;
+        hipError_t e = hipGetDeviceProperties(&prop, dev);
+        if (e != hipSuccess) return e;
+        it = g_props.emplace(dev, prop).first;
+    }
+    out = it->second;
+    return hipSuccess;
+}
 
 // The launch plan of one solve of n_jobs jobs under options o: the per-wave layout and
 // solver options (into a) and the shape (where the job state sits, persistent wavefronts,
@@ -59,7 +114,7 @@ int solve_plan(const DeviceBatch& d, const gpsat_solve_opts& o, int32_t n_jobs, 
     const size_t lds_all = state_b + 4 * (size_t)((clause_ints + 63) & ~(int64_t)63);
 
     hipDeviceProp_t prop;
-    hipError_t e = hipGetDeviceProperties(&prop, d.device);
+    hipError_t e = device_props(d.device, prop);
     if (e != hipSuccess) return hip_fail("hipGetDeviceProperties", e);
     p.cus = prop.multiProcessorCount;
     const size_t lds_cu = prop.maxSharedMemoryPerMultiProcessor ? prop.maxSharedMemoryPerMultiProcessor : 65536;
@@ -247,5 +302,6 @@ extern "C" int gpsat_solve_ref(gpsat_batch* b, const uint8_t* dead, int32_t max_
         HIP_OR(hipMemcpyAsync(model_rows, d_models, sizeof(uint32_t) * d.total_words, hipMemcpyDeviceToHost, st));
     if (stats) HIP_OR(hipMemcpyAsync(stats, d_stats, sizeof(int64_t) * N * 5, hipMemcpyDeviceToHost, st));
     HIP_OR(hipStreamSynchronize(st));
+    mem.mark_synced();
     return GPSAT_OK;
 }

@@ -254,7 +254,10 @@ int gpsat_solve_opts_default(gpsat_solve_opts *o);
  * SAT model was re-checked on the GPU); job_stats[n_jobs*GPSAT_JOB_STATS] (status, conflicts,
  * decisions, propagations, learned, restarts, max level, job time in ns, then with telemetry the
  * ns spent in set-up, BCP, conflict analysis + learning, backtracking, decisions); var_bumps (per var, how
- * often conflict analysis met it).  Every output may be NULL except status. */
+ * often conflict analysis met it).  Every output may be NULL except status.
+ * Device memory: the call's buffers come from hipMalloc or from blocks earlier solves left
+ * behind; when a call ends they are kept for reuse (up to 1 GiB per device, held until the
+ * process ends), so repeating a solve of one shape allocates nothing after the first call. */
 int gpsat_solve(gpsat_batch *b, const gpsat_solve_opts *o, const int32_t *job_inst, const int32_t *cube_off,
                 const int32_t *cube_lits, int32_t n_jobs, const float *act_init, const int8_t *phase_init,
                 int32_t *status, uint32_t *model_rows, int32_t *verified, int64_t *job_stats, int32_t *var_bumps,
