@@ -316,13 +316,14 @@ __global__ __launch_bounds__(256) void k_eval_bcp(const InstMeta* __restrict__ m
 // find a slot's partner l ^ 1 by flipping one address bit).
 // Byte address of literal slot FIELD (0-2) of a packed clause word p in a table of
 // (1 << SHIFT)-byte slots at base: one bit-field extract and one shift-add (written out,
-// since the compiler otherwise turns the pair into a shift, a mask and an add).
+// since the compiler otherwise turns the pair into a shift, a mask and an add).  The base is
+// wave-uniform and read from its scalar register (a vector operand cost one v_mov per use).
 template <int FIELD, int SHIFT>
 __device__ __forceinline__ uint32_t slot_addr(uint32_t base, uint32_t p)
 {
     uint32_t t, r;
     asm("v_bfe_u32 %0, %1, %2, 10" : "=v"(t) : "v"(p), "i"(10 * FIELD));
-    asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(r) : "v"(t), "i"(SHIFT), "v"(base));
+    asm("v_lshl_add_u32 %0, %1, %2, %3" : "=v"(r) : "v"(t), "i"(SHIFT), "s"(base));
     return r;
 }
 
@@ -357,6 +358,14 @@ __device__ __forceinline__ uint32_t and3(uint32_t a, uint32_t b, uint32_t c)
 {
     uint32_t d;
     asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x80" : "=v"(d) : "v"(a), "v"(b), "v"(c));
+    return d;
+}
+// a & ~b & ~c in one instruction (the compiler otherwise forms ~(b | c) and then the AND, and
+// keeps the complement live for the second use)
+__device__ __forceinline__ uint32_t andn2(uint32_t a, uint32_t b, uint32_t c)
+{
+    uint32_t d;
+    asm("v_bitop3_b32 %0, %1, %2, %3 bitop3:0x10" : "=v"(d) : "v"(a), "v"(b), "v"(c));
     return d;
 }
 
@@ -592,10 +601,12 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                 };
                 const auto group = [&](const uint32_t (&s)[3], const u32x4& qa, const u32x4& qb, const u32x4& qc) {
                     const uint64_t Fa = u64(qa.z, qa.w), Fb = u64(qb.z, qb.w), Fc = u64(qc.z, qc.w);
-                    const uint64_t T = u64(qa.x, qa.y) | u64(qb.x, qb.y) | u64(qc.x, qc.y);
-                    const uint64_t open = ~(T | frozen);
-                    const uint64_t newconf = u64(and3(qa.z, qb.z, qc.z), and3(qa.w, qb.w, qc.w)) & open;
-                    const uint64_t U = u64(two3(qa.z, qb.z, qc.z), two3(qa.w, qb.w, qc.w)) & open;
+                    // per 32-bit half: T = some literal true; new conflicts and units where the
+                    // row is open (neither T nor frozen), each one v_bitop3
+                    const uint32_t Tl = qa.x | qb.x | qc.x, Th = qa.y | qb.y | qc.y;
+                    const uint32_t fl = (uint32_t)frozen, fh = (uint32_t)(frozen >> 32);
+                    const uint64_t newconf = u64(andn2(and3(qa.z, qb.z, qc.z), Tl, fl), andn2(and3(qa.w, qb.w, qc.w), Th, fh));
+                    const uint64_t U = u64(andn2(two3(qa.z, qb.z, qc.z), Tl, fl), andn2(two3(qa.w, qb.w, qc.w), Th, fh));
                     if (U) {
                         const uint64_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
                         if (Ua) {
@@ -818,8 +829,8 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                 const auto group = [&](const uint32_t (&s)[3], const uint2& qa, const uint2& qb, const uint2& qc) {
                     const uint32_t Fa = qa.y, Fb = qb.y, Fc = qc.y;
                     const uint32_t T = qa.x | qb.x | qc.x;
-                    const uint32_t newconf = and3(Fa, Fb, Fc) & ~(T | fz);
-                    const uint32_t U = two3(Fa, Fb, Fc) & ~(T | fz);
+                    const uint32_t newconf = andn2(and3(Fa, Fb, Fc), T, fz);
+                    const uint32_t U = andn2(two3(Fa, Fb, Fc), T, fz);
                     if (U) {
                         const uint32_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
                         if (Ua) {
@@ -1203,9 +1214,9 @@ __device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int
             if ((frozen & amask) != amask) pk_sweep<K, false>(ipk, m, G, wv, NW, lane, tabb, zw, [&](const PkGroup<K>& gr) {
                 uint32_t T[2], N0[2], N1[2];
                 pk_status<K>(gr, T, N0, N1);
-                const uint64_t open = ~(u64(T[0], T[1]) | frozen);
-                const uint64_t newconf = u64(N0[0], N0[1]) & open;
-                const uint64_t U = u64(N1[0], N1[1]) & open;
+                const uint32_t fl = (uint32_t)frozen, fh = (uint32_t)(frozen >> 32);
+                const uint64_t newconf = u64(andn2(N0[0], T[0], fl), andn2(N0[1], T[1], fh));
+                const uint64_t U = u64(andn2(N1[0], T[0], fl), andn2(N1[1], T[1], fh));
                 if (U) {
                     // the unit literal is the one slot that is not false
 #pragma unroll
@@ -1312,9 +1323,8 @@ __device__ __forceinline__ void eval_bcpk_body(const InstMeta& mi, int inst, int
                     pk_sweep<K, false, PkGroup32<K>>(ipk, m, G, wv, NW, lane, ctb, zw, [&](const PkGroup32<K>& gr) {
                         uint32_t T, N0, N1;
                         pk_status32<K>(gr, T, N0, N1);
-                        const uint32_t open = ~(T | fz);
-                        const uint32_t newconf = N0 & open;
-                        const uint32_t U = N1 & open;
+                        const uint32_t newconf = andn2(N0, T, fz);
+                        const uint32_t U = andn2(N1, T, fz);
                         if (U) {
 #pragma unroll
                             for (int k = 0; k < K; k++) {
