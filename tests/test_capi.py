@@ -72,8 +72,9 @@ def test_struct_mirrors_match_the_header(tmp_path):
 
 
 def test_integration_example_compiles(tmp_path):
-    """The C++ caller shown in INTEGRATION.md is type-correct against include/gpsat.h (compiled
-    with the host compiler inside a function that declares its inputs; not run)."""
+    """The C++ caller example of the repository's integration guide (its first cpp block) is
+    type-correct against include/gpsat.h (compiled with the host compiler inside a function
+    that declares its inputs; not run)."""
     import shutil
     import subprocess
     cxx = shutil.which("g++") or shutil.which("c++")

@@ -361,6 +361,22 @@ def restart_points(first, factor, upto):
         thr = int(np.float32(thr) * np.float32(factor))
 
 
+def test_repeated_solves_on_reused_blocks(orc):
+    """One batch solved twice in a row (the second call on the device blocks the first left
+    for reuse), then a batch of another shape, then the first again: every call gives the
+    oracle's verdicts and GPU-verified models, so no stale block contents reach a result."""
+    print("[solve] repeated solves on reused device blocks", flush=True)
+    b1 = W.random_3sat_batch(300, 50, 213, seed=21)
+    b2 = W.random_3sat_batch(120, 60, 256, seed=22)
+    exp1 = [truth(orc, b1.instance(i)) for i in range(b1.n_inst)]
+    exp2 = [truth(orc, b2.instance(i)) for i in range(b2.n_inst)]
+    with G.GpuBatch(b1) as g1, G.GpuBatch(b2) as g2:
+        for gb, b, exp in ((g1, b1, exp1), (g1, b1, exp1), (g2, b2, exp2), (g1, b1, exp1)):
+            res = gb.solve(opts())
+            check_models(orc, gb, res, b)
+            assert res["status"].tolist() == exp
+
+
 @pytest.mark.parametrize("first,factor", [(100, 1.3), (40, 2.0)])
 def test_restart_schedule(orc, first, factor):
     """k_solve's restart count follows the geometric schedule: restart k needs the sum of the
