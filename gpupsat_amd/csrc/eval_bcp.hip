@@ -610,19 +610,15 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                     const uint64_t newconf = u64(andn2(and3(qa.z, qb.z, qc.z), Tl, fl), andn2(and3(qa.w, qb.w, qc.w), Th, fh));
                     const uint64_t U = u64(andn2(two3(qa.z, qb.z, qc.z), Tl, fl), andn2(two3(qa.w, qb.w, qc.w), Th, fh));
                     if (U) {
+                        // all six ORs for a lane with a unit clause (a literal that is not the
+                        // unit one ORs zero): one branch instead of one per literal
                         const uint64_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
-                        if (Ua) {
-                            lds_or64(s[0], Ua);
-                            lds_or64((s[0] ^ 16u) + 8u, Ua);
-                        }
-                        if (Ub) {
-                            lds_or64(s[1], Ub);
-                            lds_or64((s[1] ^ 16u) + 8u, Ub);
-                        }
-                        if (Uc) {
-                            lds_or64(s[2], Uc);
-                            lds_or64((s[2] ^ 16u) + 8u, Uc);
-                        }
+                        lds_or64(s[0], Ua);
+                        lds_or64((s[0] ^ 16u) + 8u, Ua);
+                        lds_or64(s[1], Ub);
+                        lds_or64((s[1] ^ 16u) + 8u, Ub);
+                        lds_or64(s[2], Uc);
+                        lds_or64((s[2] ^ 16u) + 8u, Uc);
                     }
                     chg |= U;
                     if (ballot(newconf != 0)) {
@@ -835,18 +831,12 @@ __global__ __launch_bounds__(256) void k_eval_bcp3(const InstMeta* __restrict__ 
                     const uint32_t U = andn2(two3(Fa, Fb, Fc), T, fz);
                     if (U) {
                         const uint32_t Ua = U & ~Fa, Ub = U & ~Fb, Uc = U & ~Fc;
-                        if (Ua) {
-                            lds_or32(s[0], Ua);
-                            lds_or32((s[0] ^ 8u) + 4u, Ua);
-                        }
-                        if (Ub) {
-                            lds_or32(s[1], Ub);
-                            lds_or32((s[1] ^ 8u) + 4u, Ub);
-                        }
-                        if (Uc) {
-                            lds_or32(s[2], Uc);
-                            lds_or32((s[2] ^ 8u) + 4u, Uc);
-                        }
+                        lds_or32(s[0], Ua);
+                        lds_or32((s[0] ^ 8u) + 4u, Ua);
+                        lds_or32(s[1], Ub);
+                        lds_or32((s[1] ^ 8u) + 4u, Ub);
+                        lds_or32(s[2], Uc);
+                        lds_or32((s[2] ^ 8u) + 4u, Uc);
                     }
                     chg |= U;
                     if (ballot(newconf != 0)) {
