@@ -307,43 +307,85 @@ int gpsat_autotune_eval(gpsat_batch* b, const uint32_t* rows_dev, int32_t A, int
     float best_ms = 1e30f;
     rep.line("autotune eval/BCP: %d instances, A=%d, eval=%d bcp=%d, %d launches per shape", d.n_inst, A, do_eval, do_bcp,
              reps);
-    for (int fast : {1, 0}) {
-        if (rc != GPSAT_OK) break;
-        if (fast && !d.fast3) continue;
-        for (int stage : {0, 1}) {
-            if (fast && stage) continue;     // the register-resident kernel has no staged arena
-            for (int wpb : {1, 2, 4}) {
-                tuning().eval_fast3 = fast;
-                tuning().eval_stage_arena = stage;
-                tuning().eval_waves_per_block = wpb;
-                auto launch = [&]() {
-                    return launch_eval_bcp(d, rows_dev, A, (int32_t*)counts, (uint32_t*)out, (int32_t*)conf,
-                                           (int32_t*)props, nullptr, do_eval != 0, do_bcp != 0, d.stream);
-                };
-                e = launch();     // warm-up
-                if (e == hipSuccess) e = hipEventRecord(e0, d.stream);
-                for (int r = 0; r < reps && e == hipSuccess; r++) e = launch();
-                if (e == hipSuccess) e = hipEventRecord(e1, d.stream);
-                if (e == hipSuccess) e = hipEventSynchronize(e1);
-                float ms = 0;
-                if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-                if (e != hipSuccess) {
-                    fail("autotune_eval launch", e);
-                    break;
-                }
-                ms /= reps;
-                rep.line("  kernel=%s stage=%d waves/workgroup=%d: %.4f ms", fast ? "3-literal" : "general", stage, wpb, ms);
-                if (ms < best_ms) {
-                    best_ms = ms;
-                    best = tuning();
-                }
+    // Candidate shapes per kernel the batch can take: the 3-literal kernel (instances per
+    // workgroup, the small-row threshold of its BCP), the packed general kernel (wavefronts
+    // per instance, the 32-bit handoff on or off) and the CSR kernel (staged arena, instances
+    // per workgroup).  Each is timed as the batch would launch it.
+    static const char* const kKind[] = {"3-literal", "packed", "CSR", "CSR (HBM tables)"};
+    std::vector<Tuning> cands;
+    if (d.fast3)
+        for (int wpb : {1, 2, 4})
+            for (int s64 : {0, 8, 32}) {
+                Tuning t = saved;
+                t.eval_fast3 = 1;
+                t.eval_waves_per_block = wpb;
+                t.eval_fast3_small64 = s64;
+                cands.push_back(t);
             }
+    if (d.pkk)
+        for (int nw : {1, 2, 3, 4})
+            for (int cmax : {0, 32}) {
+                Tuning t = saved;
+                t.eval_fast3 = 0;
+                t.eval_pkk = 1;
+                t.eval_pkk_waves = nw;
+                t.eval_pkk_compact = cmax;
+                cands.push_back(t);
+            }
+    for (int stage : {0, 1})
+        for (int wpb : {1, 2, 4}) {
+            Tuning t = saved;
+            t.eval_fast3 = 0;
+            t.eval_pkk = 0;
+            t.eval_stage_arena = stage;
+            t.eval_waves_per_block = wpb;
+            cands.push_back(t);
+        }
+    const auto describe = [&](const Tuning& t, char* buf, size_t len) {
+        tuning() = t;
+        const EvalKernel ek = eval_kernel(d);
+        if (ek == EK_FAST3)
+            snprintf(buf, len, "kernel=%s waves/workgroup=%d small64=%d", kKind[ek], t.eval_waves_per_block,
+                     t.eval_fast3_small64);
+        else if (ek == EK_PACKED)
+            snprintf(buf, len, "kernel=%s waves/instance=%d handoff rows=%d", kKind[ek], t.eval_pkk_waves,
+                     t.eval_pkk_compact);
+        else
+            snprintf(buf, len, "kernel=%s stage=%d waves/workgroup=%d", kKind[ek], t.eval_stage_arena,
+                     t.eval_waves_per_block);
+    };
+    char what[128];
+    for (const Tuning& t : cands) {
+        if (rc != GPSAT_OK) break;
+        tuning() = t;
+        auto launch = [&]() {
+            return launch_eval_bcp(d, rows_dev, A, (int32_t*)counts, (uint32_t*)out, (int32_t*)conf, (int32_t*)props,
+                                   nullptr, do_eval != 0, do_bcp != 0, d.stream);
+        };
+        e = launch();     // warm-up
+        if (e == hipSuccess) e = hipEventRecord(e0, d.stream);
+        for (int r = 0; r < reps && e == hipSuccess; r++) e = launch();
+        if (e == hipSuccess) e = hipEventRecord(e1, d.stream);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float ms = 0;
+        if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+        if (e != hipSuccess) {
+            fail("autotune_eval launch", e);
+            break;
+        }
+        ms /= reps;
+        describe(t, what, sizeof(what));
+        rep.line("  %s: %.4f ms", what, ms);
+        if (ms < best_ms) {
+            best_ms = ms;
+            best = t;
         }
     }
+    if (rc == GPSAT_OK) {
+        describe(best, what, sizeof(what));
+        rep.line("chosen: %s (%.4f ms)", what, best_ms);
+    }
     tuning() = rc == GPSAT_OK ? best : saved;
-    if (rc == GPSAT_OK)
-        rep.line("chosen: kernel=%s stage=%d waves/workgroup=%d (%.4f ms)", best.eval_fast3 && d.fast3 ? "3-literal" : "general",
-                 best.eval_stage_arena, best.eval_waves_per_block, best_ms);
     rep.copy_to(report, report_len);
     if (e0) (void)hipEventDestroy(e0);
     if (e1) (void)hipEventDestroy(e1);

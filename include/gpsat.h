@@ -309,8 +309,11 @@ int gpsat_solve_ref(gpsat_batch *b, const uint8_t *dead, int32_t max_iterations,
  * clause_mem and chunk of the shape with the most propagations per ms of kernel time.
  * Verdicts do not depend on the shape.  report (optional) receives one line per trial.
  * gpsat_autotune_eval: times reps launches (0: 10) of clause evaluation and/or BCP on the
- * caller's rows under each eval/BCP shape ("eval_fast3", "eval_stage_arena",
- * "eval_waves_per_block") and leaves the tuning keys at the fastest. */
+ * caller's rows under each shape of every kernel the batch can take -- the 3-literal kernel
+ * ("eval_waves_per_block" x "eval_fast3_small64"), the packed general kernel
+ * ("eval_pkk_waves" x "eval_pkk_compact" 0/32) and the CSR kernel ("eval_stage_arena" x
+ * "eval_waves_per_block") -- and leaves the tuning keys at the fastest.  Outputs do not
+ * depend on the shape. */
 int gpsat_autotune_solve(gpsat_batch *b, const gpsat_solve_opts *base, const int32_t *job_inst, const int32_t *cube_off,
                          const int32_t *cube_lits, int32_t n_jobs, int32_t sample_jobs, double trial_ms,
                          gpsat_solve_opts *best, char *report, int32_t report_len);

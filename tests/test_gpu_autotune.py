@@ -14,13 +14,46 @@ def test_autotune_eval_keeps_parity(orc):
     b = W.random_3sat_batch(512, 200, 852, seed=21)
     A = 64
     rows = W.random_assignments(b, A, 0.1, seed=22)
-    keys = ("eval_fast3", "eval_stage_arena", "eval_waves_per_block")
+    keys = ("eval_fast3", "eval_pkk", "eval_stage_arena", "eval_waves_per_block", "eval_fast3_small64",
+            "eval_pkk_waves", "eval_pkk_compact")
     saved = {k: G.tune_get(k) for k in keys}
     try:
         with G.GpuBatch(b) as gb:
             rb = G.DevBuf.from_host(rows)
             rep = gb.autotune_eval(rb, A, reps=3)
-            assert "chosen:" in rep and rep.count("ms") >= 4
+            # 9 shapes of the 3-literal kernel and 6 of the CSR kernel, each timed
+            assert "chosen:" in rep and rep.count("kernel=3-literal") >= 9 and rep.count("kernel=CSR") >= 6
+            counts = gb.clause_eval(rows, A)
+            out, conflict, props = gb.bcp_fixpoint(rows, A)
+        np.testing.assert_array_equal(counts, orc.eval_batch(b, rows, A))
+        e_out, e_conf, e_props = orc.bcp_batch(b, rows, A)
+        np.testing.assert_array_equal(conflict, e_conf)
+        np.testing.assert_array_equal(props, e_props)
+        np.testing.assert_array_equal(out, e_out)
+    finally:
+        for k, v in saved.items():
+            G.tune_set(k, v)
+
+
+def test_autotune_eval_packed_shapes(orc):
+    """On a mixed k-SAT batch the eval autotuner times the packed kernel's shapes (waves per
+    instance 1-4, handoff to 32-bit masks off and on) beside the CSR kernel's, and the
+    fixpoints under the chosen shape are the oracle's."""
+    print("[autotune] eval/BCP shapes on 48 mixed k-SAT", flush=True)
+    mixed, ks = W.mixed_ksat_batch(4096, seed=3)
+    idx = [int(i) for k in (3, 4, 5) for i in np.nonzero(ks == k)[0][:16]]
+    from gpupsat_amd.formula import Batch
+    b = Batch.from_formulas([mixed.instance(i) for i in idx])
+    A = 64
+    rows = W.random_assignments(b, A, 0.3, seed=23)
+    keys = ("eval_fast3", "eval_pkk", "eval_stage_arena", "eval_waves_per_block", "eval_fast3_small64",
+            "eval_pkk_waves", "eval_pkk_compact")
+    saved = {k: G.tune_get(k) for k in keys}
+    try:
+        with G.GpuBatch(b) as gb:
+            rb = G.DevBuf.from_host(rows)
+            rep = gb.autotune_eval(rb, A, reps=2)
+            assert "chosen:" in rep and rep.count("kernel=packed") >= 8 and rep.count("kernel=CSR") >= 6
             counts = gb.clause_eval(rows, A)
             out, conflict, props = gb.bcp_fixpoint(rows, A)
         np.testing.assert_array_equal(counts, orc.eval_batch(b, rows, A))
